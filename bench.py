@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Swift SpMM benchmark on one MI355X.
+
+Workload: the 10^6 x 10^6, 16M-nonzero matrix (even rows banded with
+bandwidth 64 and 8..32 nonzeros, odd rows Zipf(1.8) lengths up to 20k with
+uniform columns, values N(0,1)), FP32, K = 128, row-major B (uniform in
+[-1,1], original column order) and C.  One step = one C = A*B (zeroing C, the
+regular tile kernel and the irregular reside kernel) with A's Swift format and
+B already resident in HBM.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the regular
+tile kernel): algorithmic bytes per launch (CSR-equivalent A read once, B read
+once, C written once) over its average launch time, timed with HIP events on
+the launch stream.  `cpu_baseline` times the reference's own Swift_cpu
+(oracle/_ref, compiled from the reference sources) on the same matrix.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "swift-high-performance-sparse-dense-matrix-multiplication-on-gpus_amd")
+sys.path[:0] = [PKG, os.path.join(ROOT, "oracle")]
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(m, n, nnz, K, s_a=4, s_b=4, s_c=4):
+    """nnz*(4 + s_A) + 4*(M+1) + N*K*s_B + M*K*s_C."""
+    return nnz * (4 + s_a) + 4 * (m + 1) + n * K * s_b + m * K * s_c
+
+
+def cpu_baseline(d, K, seed):
+    """Reference Swift_cpu (src/Swift_cpu.h:31-65), 1 thread, on the same matrix."""
+    import oracle as O
+    m, n, nnz = d["m"], d["n"], d["nnz"]
+    vals = d["val"].astype(np.float32)
+    f = O.build(m, n, d["rowptr"], d["colidx"], vals.astype(np.float64))
+    B = np.random.default_rng(seed).uniform(-1, 1, (n, K)).astype(np.float32)
+    sB = np.ascontiguousarray(B[f["perm"]])
+    del B
+    if O.Ref.available(np.float32):
+        ref = O.Ref(np.float32)
+        h, _ = ref.format(m, n, f["sortrowidx_tmp"], f["sortval_tmp"].astype(np.float32),
+                          f["sortnnz"])
+        _, ms = ref.swift_cpu(h, m, n, nnz, sB)
+        ref.free(h)
+        kind = "reference"
+    else:
+        _, ms = f.swift_cpu_f32_timed(K, sB)
+        kind = "port"
+    gflops = 2.0 * nnz * K / (ms * 1e6)
+    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": 1, "kind": kind,
+            "sample": "whole workload once: %dx%d, %d nnz, K=%d, FP32, Swift_cpu loops "
+                      "(%.0f ms, gettimeofday around the loops as the reference does)"
+                      % (m, n, nnz, K, ms)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink the matrix (tests only)")
+    ap.add_argument("--K", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--extras", action="store_true", help="also time the other forms")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the Swift SpMM path runs on a single MI355X "
+              "(no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import swift_amd as S
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    K, seed = args.K, 0
+    t0 = time.time()
+    d = S.generate(2, scale=args.scale, seed=seed)
+    m, n, nnz = d["m"], d["n"], d["nnz"]
+    log("matrix %dx%d nnz=%d generated in %.1fs" % (m, n, nnz, time.time() - t0))
+    vals32 = d["val"].astype(np.float32)
+
+    torch.cuda.synchronize()
+    t0 = time.time()
+    M = S.SwiftMatrix(m, n, d["rowptr"], d["colidx"], vals32, dtype=torch.float32)
+    torch.cuda.synchronize()
+    build_ms = (time.time() - t0) * 1e3
+    inf = M.info()
+    log("swift_build %.1f ms: tilenum=%d reside_nnz=%d numblk=%d nsteps=%d" %
+        (build_ms, inf.tilenum, inf.reside_nnz, inf.numblk, inf.nsteps))
+
+    g = torch.Generator(device=dev).manual_seed(seed)
+    B = torch.rand((n, K), device=dev, generator=g, dtype=torch.float32) * 2 - 1
+    C = torch.empty((m, K), device=dev, dtype=torch.float32)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        M.spmm(B, out=C)
+    torch.cuda.synchronize()
+
+    # whole-job throughput: K steps bracketed by synchronisation
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        M.spmm(B, out=C)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_time(e1) / args.steps
+    flops = 2.0 * nnz * K
+    value = flops / (ms_step * 1e-3) / 1e9
+    log("spmm: %.3f ms/step (events), wall %.3f ms/step, %.1f GFLOP/s" %
+        (ms_step, wall * 1e3 / args.steps, value))
+
+    # per-kernel launch times (HIP events between the launches, same stream)
+    phases = np.array([M.spmm_timed(B, C) for _ in range(max(5, min(args.steps, 20)))])
+    zero_ms, reg_ms, res_ms = [float(x) for x in np.median(phases, axis=0)]
+    log("phases (median ms): zero %.4f regular %.4f reside %.4f" % (zero_ms, reg_ms, res_ms))
+    alg = algorithmic_bytes(m, n, nnz, K)
+    achieved = alg / (reg_ms * 1e-3)
+    out = {
+        "metric": "SpMM GFLOP/s + % HBM roofline (K=128 fp32); bf16 & fwd+bwd vs plain fp32",
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, seed 0)",
+        "config": {"workload": "Swift SpMM C=A*B, A %dx%d with %d nnz (even rows banded bw 64 "
+                               "8..32 nnz, odd rows Zipf(1.8) <=20k), FP32, K=%d, row-major B "
+                               "U[-1,1] in original column order" % (m, n, nnz, K),
+                   "rows": m, "cols": n, "nnz": nnz, "K": K, "layout": "row-major",
+                   "build_ms": round(build_ms, 1),
+                   "phase_ms": {"zero_C": round(zero_ms, 4), "regular": round(reg_ms, 4),
+                                "reside": round(res_ms, 4)}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "kernel": "k_regular_row (regular tiles)",
+                     "algorithmic_bytes": alg},
+        "gpu": torch.cuda.get_device_name(0),
+    }
+    if args.extras:
+        out["extras"] = extras(S, torch, M, d, B, K)
+    M.close()
+    del B, C
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        t0 = time.time()
+        out["cpu_baseline"] = cpu_baseline(d, K, seed)
+        out["cpu_baseline"]["cpu"] = platform.processor() or platform.machine()
+        log("cpu baseline %.1fs: %s" % (time.time() - t0, out["cpu_baseline"]))
+    print(json.dumps(out), flush=True)
+
+
+def time_call(torch, fn, reps=10, warm=2):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def extras(S, torch, M, d, B, K):
+    """Other forms on the same matrix: layouts, sorted B, BF16 storage, FP64."""
+    m, n, nnz = d["m"], d["n"], d["nnz"]
+    flops = 2.0 * nnz * K
+    res = {}
+    C = torch.empty((m, K), device="cuda", dtype=torch.float32)
+    base = time_call(torch, lambda: M.spmm(B, out=C))
+    res["f32_row_ms"] = round(base, 4)
+    perm = torch.tensor(M.arrays(["perm"])["perm"].astype(np.int64), device="cuda")
+    Bs = B[perm].contiguous()
+    res["f32_row_Bsorted_ms"] = round(time_call(torch, lambda: M.spmm(Bs, out=C,
+                                                                      flags=S.B_SORTED)), 4)
+    Bt = B.t().contiguous()
+    Ct = torch.empty((K, m), device="cuda", dtype=torch.float32)
+    res["f32_col_ms"] = round(time_call(torch, lambda: M.spmm(Bt, out=Ct, layout=S.COL_MAJOR),
+                                        reps=3, warm=1), 4)
+    M.prepare_values(S.BF16)
+    Bh = S.convert_dense(B, S.BF16)
+    bf = time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16))
+    res["bf16_row_ms"] = round(bf, 4)
+    res["bf16_speedup_vs_f32"] = round(base / bf, 3)
+    dC = torch.randn((m, K), device="cuda")
+    bw = time_call(torch, lambda: M.backward_fused(dC, B), reps=5)
+    res["fwd_bwd_over_fwd"] = round((base + bw) / base, 3)
+    res["f32_row_gflops"] = round(flops / base / 1e6, 1)
+    return res
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,271 @@
+/*
+ * swift_spmm.h -- C ABI of the MI355X (gfx950) Swift SpMM library.
+ *
+ * The reference exposes its path as C++ header functions called in a fixed
+ * order by src/main.cu (load .mtx -> csr_to_csc -> col sort ->
+ * formattransation -> block_catgory -> balance -> GPU SpMM launchers).  This
+ * header replaces that interface with plain C entry points: plain pointers and
+ * sizes, explicit value type / layout / K, status-code returns and an optional
+ * HIP stream.  Device pointers are named d_*; everything else is host memory.
+ *
+ * Value types: the matrix is built in F32 or F64 (the reference's float and
+ * double forks).  SpMM additionally accepts BF16 and F16 storage of A's values
+ * and of B with FP32 accumulation and an FP32 C.
+ */
+#ifndef SWIFT_SPMM_H
+#define SWIFT_SPMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *swift_stream_t; /* == hipStream_t */
+
+typedef enum {
+    SWIFT_OK = 0,
+    SWIFT_ERR_OPEN = -1,        /* mmio_allinone: fopen failed          */
+    SWIFT_ERR_BANNER = -2,      /* mmio_allinone: bad banner            */
+    SWIFT_ERR_DATA = -3,        /* .mtx entry lines malformed/truncated */
+    SWIFT_ERR_SIZE = -4,        /* mmio_allinone: bad size line         */
+    SWIFT_ERR_ARG = -10,        /* invalid argument                     */
+    SWIFT_ERR_HIP = -11,        /* HIP runtime error (see last_error)   */
+    SWIFT_ERR_ALLOC = -12,      /* device or host allocation failed     */
+    SWIFT_ERR_UNSUPPORTED = -13,/* dtype/layout/K combination not built */
+    SWIFT_ERR_VERIFY = -14      /* host launcher: result check failed   */
+} swift_status;
+
+typedef enum { SWIFT_F32 = 0, SWIFT_F64 = 1, SWIFT_BF16 = 2, SWIFT_F16 = 3 } swift_dtype;
+
+/* Dense B (n x K) and C (m x K) storage order.  ROW: B[j*K + k];
+ * COL: B[k*n + j] (the reference's "column-major", src/Swift.h:43-44). */
+typedef enum { SWIFT_ROW_MAJOR = 0, SWIFT_COL_MAJOR = 1 } swift_layout;
+
+/* Kernel family.  DEFAULT picks the tuned kernel for the layout. */
+typedef enum {
+    SWIFT_KERNEL_DEFAULT = 0,
+    SWIFT_KERNEL_BALANCED = 1,   /* reside part per balance block (Swift_GPU / Swift_gpu2) */
+    SWIFT_KERNEL_UNBALANCED = 2, /* reside part per column (Swift_gpu1)                    */
+    SWIFT_KERNEL_KCHUNK = 3,     /* 2-D grid tiles x K-chunks (Swift_GPU_3)                */
+    SWIFT_KERNEL_ROWSORT = 4     /* same-row segmented reduction over row-sorted groups
+                                    of 8 tiles (Swift_GPU_4); needs swift_row_idx_sort */
+} swift_kernel;
+
+/* swift_spmm flags */
+#define SWIFT_B_SORTED 0x1u   /* B rows (ROW) / columns (COL) already in sorted
+                                 column order, i.e. the reference's sort_dense_mtx */
+#define SWIFT_ACCUMULATE 0x2u /* C += A*B instead of C = A*B                     */
+
+typedef struct swift_matrix swift_matrix; /* opaque, device resident */
+
+/* Sizes and device pointers of every array of the Swift format.  Field names
+ * follow slide_matrix (src/format.h:6-39); pointers are device pointers owned
+ * by the handle. */
+typedef struct {
+    int m, n, nnz;
+    swift_dtype vtype;
+    int tilenum, regular_nnz, reside_col, reside_nnz, segsum, numblk, nsteps;
+    int has_rowsort;
+    const int *perm, *sortnnz, *sortrowidx_tmp;
+    const void *sortval_tmp;
+    const int *tile_ptr, *tile_len, *tile_colidx, *tile_tall, *tile_format;
+    const int *sortrowidx;
+    const void *sortval;
+    const int *countrow, *segmentoffset, *sortrowindex;
+    const int *reside_cscptr, *reside_cscrowidx;
+    const void *reside_val;
+    const unsigned int *colorblk;
+    const int *blkstart, *blkstop, *blkptr;
+    const int *val_position, *val_offset, *rowsorted;
+    const int *sortsrc, *reside_src; /* CSR position of each entry (ours) */
+} swift_info;
+
+/* ------------------------------------------------------------------ host */
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *swift_last_error(void);
+/* Library version string. */
+const char *swift_version(void);
+
+/* Matrix Market loader.  Replaces mmio_allinone (src/auxiliary.h:167): .mtx
+ * -> 0-based CSR, symmetric/hermitian mirrored, pattern -> 1.0, within-row
+ * order = file order.  Arrays are malloc'd; release with swift_host_free.
+ * Returns 0, -1 (open), -2 (banner), -3 (entries), -4 (size line). */
+int swift_mmio_allinone(int *m, int *n, int *nnz, int *isSymmetric,
+                        int **csrrowptr, int **csrcolidx, double **csrval,
+                        const char *filename);
+int swift_mmio_allinone_f32(int *m, int *n, int *nnz, int *isSymmetric,
+                            int **csrrowptr, int **csrcolidx, float **csrval,
+                            const char *filename);
+void swift_host_free(void *p);
+
+/* Seeded synthetic inputs of the benchmark configurations (host memory,
+ * malloc'd, release with swift_host_free).  which:
+ *   1 = 4096^2, ~80k nnz, Zipf(2.0) row lengths in [1,512], 30% of rows as
+ *       8-wide column runs, values U[-1,1]            (k_hint 32)
+ *   2 = 10^6^2, 16M nnz, even rows banded (bandwidth 64, 8..32 nnz), odd
+ *       rows Zipf(1.8) up to 20k, values N(0,1)       (k_hint 128)
+ *   4 = symmetric Chung-Lu graph, 200k nodes, mean degree 50, degree
+ *       exponent 2.1, self loops, values D^-1/2 (A+I) D^-1/2 (k_hint 128)
+ *   5 = batch item `index`: square, rows log-U[1k,64k], mean nnz/row
+ *       log-U[4,128], Zipf(2.0) row lengths, N(0,1)   (k_hint from {32,64,128})
+ * scale in (0,1] shrinks the row/column counts (1 = full size).  Values are
+ * float64; k_hint (may be NULL) receives the configuration's K. */
+int swift_generate(int which, int index, double scale, uint64_t seed, int *m,
+                   int *n, int *nnz, int **rowptr, int **colidx, double **val,
+                   int *k_hint);
+/* Dense matrix of rows x cols, uniform in [lo,hi) (dist 0) or N(0,1) (dist 1). */
+int swift_generate_dense(size_t count, int dist, double lo, double hi,
+                         uint64_t seed, double *out);
+
+/* ---------------------------------------------------- preprocessing (GPU) */
+
+/* Stable CSR -> CSC transpose on the GPU.  Replaces csr_to_csc
+ * (src/gpu_csr2csc.h:51) and nnzpercolSegmentReduce (:24): rows ascending
+ * inside each column, nnzpercol[j] = colptr[j+1]-colptr[j].  Values are of
+ * vtype (F32/F64); d_csc_val may be NULL to skip values. */
+int swift_csr_to_csc(int m, int n, int nnz, swift_dtype vtype,
+                     const void *d_csr_val, const int *d_csr_row_ptr,
+                     const int *d_csr_col_ind, void *d_csc_val,
+                     int *d_csc_row_ind, int *d_csc_col_ptr, int *d_nnzpercol,
+                     swift_stream_t stream);
+
+/* Stable column sort by nnz on the GPU.  Replaces ColSort
+ * (src/ColSort_cuda.h:48) and col_sort (src/coo2csc.h:82): d_perm[i] =
+ * column at sorted position i (ties by column index), d_sortnnz[i] = its nnz,
+ * CSC columns concatenated in that order.  d_B/d_sortB (n x K row-major, the
+ * dense operand, Pre4sortedx src/ColSort_cuda.h:9) may be NULL. */
+int swift_col_sort(int n, int nnz, int K, swift_dtype vtype,
+                   const int *d_nnzpercol, const int *d_csc_col_ptr,
+                   const int *d_csc_row_ind, const void *d_csc_val,
+                   int *d_sortrowidx_tmp, void *d_sortval_tmp, int *d_sortnnz,
+                   int *d_perm, const void *d_B, void *d_sortB,
+                   swift_stream_t stream);
+
+/* Whole preprocessing on the GPU: csr_to_csc -> ColSort -> formattransation
+ * (src/coo2csc.h:137) -> block_catgory (src/block_catgory.h:1) -> balance
+ * (src/balance.h:1).  Input CSR is device resident (vtype F32 or F64);
+ * nothing of it is kept except a copy of the row pointer. */
+int swift_build(swift_matrix **out, int m, int n, int nnz, swift_dtype vtype,
+                const int *d_csr_row_ptr, const int *d_csr_col_ind,
+                const void *d_csr_val, swift_stream_t stream);
+/* Same, from host CSR (copies in, builds, synchronises). */
+int swift_build_host(swift_matrix **out, int m, int n, int nnz,
+                     swift_dtype vtype, const int *csr_row_ptr,
+                     const int *csr_col_ind, const void *csr_val);
+void swift_destroy(swift_matrix *mat);
+
+/* Per-8-tile-group row sort on the GPU.  Replaces row_idx_sort
+ * (src/back_process.h:23) and its bitonic GPU twin
+ * (test/preprocess/back_process.h:56-116): fills val_position, val_offset and
+ * the sorted row keys (kept beside, not over, sortrowidx). */
+int swift_row_idx_sort(swift_matrix *mat, swift_stream_t stream);
+
+/* Sizes + device pointers of the format (valid until swift_destroy). */
+int swift_get_info(const swift_matrix *mat, swift_info *info);
+
+/* Permute dense rows into sorted column order: out[i,:] = in[perm[i],:]
+ * (ROW) or out[:,i] = in[:,perm[i]] (COL).  Pre4sortedx
+ * (src/ColSort_cuda.h:9-21) as a standalone call. */
+int swift_permute_dense(const swift_matrix *mat, int K, swift_layout layout,
+                        swift_dtype dtype, const void *d_in, void *d_out,
+                        swift_stream_t stream);
+
+/* ------------------------------------------------------------- SpMM (GPU) */
+
+/* C = A*B on the GPU (C += A*B with SWIFT_ACCUMULATE).  Replaces the
+ * launchers Swift_gpu1 (src/Swift_gpu_col_major.h:171), Swift_gpu2
+ * (src/Swift_gpu_row_major.h:136), Swift_GPU (src/Swift.h:184), Swift_GPU_3
+ * (src/Swift_3.h:216) and Swift_GPU_4 (src/Swift_4.h:305), minus their
+ * host copies: B (n x K) and C (m x K) are device resident in `layout`.
+ * dtype F32/F64 must equal the build's value type; BF16/F16 need
+ * swift_prepare_values first and take B in that type, C in F32. */
+int swift_spmm(const swift_matrix *mat, int K, swift_layout layout,
+               swift_dtype dtype, swift_kernel kernel, unsigned flags,
+               const void *d_B, void *d_C, swift_stream_t stream);
+
+/* swift_spmm with HIP events recorded on `stream` between its launches, the
+ * way the reference times only its kernels (src/Swift.h:298-336, where C is
+ * zeroed before the timed region): kernel_ms[0] = zeroing of C, [1] = regular
+ * (tile) kernels, [2] = irregular (reside) kernels.  Synchronises `stream`. */
+int swift_spmm_timed(const swift_matrix *mat, int K, swift_layout layout,
+                     swift_dtype dtype, swift_kernel kernel, unsigned flags,
+                     const void *d_B, void *d_C, swift_stream_t stream,
+                     float kernel_ms[3]);
+
+/* Host-pointer launcher with the reference's semantics (e.g. src/Swift.h:184):
+ * copies sorted B in, runs `kernel`, times the kernels with HIP events
+ * (median of `reps` after `warmup`), copies C out, then counts elements with
+ * |C - golden| > 1e-2 (ResultVerify, src/auxiliary.h:156); on any mismatch
+ * *ms = -1 and SWIFT_ERR_VERIFY is returned.  golden may be NULL. */
+int swift_spmm_host(const swift_matrix *mat, int K, swift_layout layout,
+                    swift_dtype dtype, swift_kernel kernel, const void *h_sortB,
+                    void *h_C, const void *h_golden, int warmup, int reps,
+                    float *ms, double *gflops);
+
+/* BF16/F16 storage of A's values (FP32 accumulation).  For F16, the values
+ * are divided by one global power-of-two scale (returned by the B conversion
+ * into `scales`).  Must precede swift_spmm with dtype BF16/F16. */
+int swift_prepare_values(swift_matrix *mat, swift_dtype dtype,
+                         swift_stream_t stream);
+/* Convert dense FP32 B (count = rows*K elements, any layout) to BF16 or F16.
+ * For F16 with d_col_scale != NULL: per-column (k) power-of-two scales are
+ * computed so that max |B[:,k]|/scale <= 2^14; layout/K give the k index. */
+int swift_convert_dense(const float *d_in, void *d_out, size_t rows, int K,
+                        swift_layout layout, swift_dtype dtype,
+                        float *d_col_scale, swift_stream_t stream);
+/* F16 path with per-column scales: C[:,k] = scale[k] * (A_f16 * B_f16)[:,k]. */
+int swift_spmm_scaled(const swift_matrix *mat, int K, swift_layout layout,
+                      swift_kernel kernel, unsigned flags, const void *d_B,
+                      const float *d_col_scale, float *d_C,
+                      swift_stream_t stream);
+
+/* -------------------------------------------------------- backward (GPU) */
+
+/* Swift layout of A^T, built once on the GPU from A's own column-sorted CSC
+ * (which is A^T's CSR); reused by every backward call. */
+int swift_build_transpose(const swift_matrix *A, swift_matrix **AT,
+                          swift_stream_t stream);
+/* dvals[e] = <dC[row_e,:], B[col_e,:]> on A's pattern, e in A's CSR order
+ * (SDDMM).  Row-major dC (m x K) and B (n x K), original column order. */
+int swift_sddmm(const swift_matrix *A, int K, swift_dtype dtype,
+                const void *d_dC, const void *d_B, void *d_dvals,
+                swift_stream_t stream);
+/* Fused backward over A's Swift layout in one pass: dB = A^T*dC and
+ * dvals = SDDMM(dC, B); dC is read once per nonzero.  Either output may be
+ * NULL.  Row-major, original column order, F32/F64. */
+int swift_backward_fused(const swift_matrix *A, int K, swift_dtype dtype,
+                         const void *d_dC, const void *d_B, void *d_dB,
+                         void *d_dvals, swift_stream_t stream);
+/* Scatter new CSR-order values into the built format (values change, pattern
+ * fixed): used when A's values are trainable. */
+int swift_update_values(swift_matrix *mat, const void *d_csr_val,
+                        swift_stream_t stream);
+
+/* -------------------------------------------------------- batched (GPU) */
+
+typedef struct swift_batch swift_batch; /* opaque */
+
+/* Group `count` built matrices (F32) with their K for one persistent launch.
+ * The autotuner picks the work-unit depth and block shape per batch from K,
+ * density and the regular/irregular ratio. */
+int swift_batch_create(swift_batch **out, swift_matrix *const *mats,
+                       const int *Ks, int count, swift_stream_t stream);
+/* All C_i = A_i * B_i (row-major, original column order, F32) in one
+ * persistent kernel; d_B[i], d_C[i] are device pointers. */
+int swift_batch_spmm(swift_batch *batch, const float *const *d_B,
+                     float *const *d_C, swift_stream_t stream);
+/* Capture swift_batch_spmm into a HIP graph once, then replay it. */
+int swift_batch_graph_launch(swift_batch *batch, const float *const *d_B,
+                             float *const *d_C, swift_stream_t stream);
+/* Tuned parameters (units, depth, block threads) for reporting. */
+int swift_batch_params(const swift_batch *batch, int *work_units,
+                       int *depth, int *block_threads);
+void swift_batch_destroy(swift_batch *batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWIFT_SPMM_H */

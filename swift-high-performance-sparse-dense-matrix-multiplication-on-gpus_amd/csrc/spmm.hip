@@ -1,0 +1,815 @@
+// spmm.hip -- Swift SpMM kernels for gfx950 (CDNA4, 64-lane waves).
+//
+// Regular part (32-wide staircase tiles) and irregular part (reside CSC split
+// into balance blocks), for row-major and column-major dense operands.
+//
+// Row-major B/C (the fast layout).  Lanes run along K: one entry's B row is
+// LPE lanes x VEC elements (16-byte loads), so a wave holds EPS = 64/LPE
+// entries at a time.  The wave walks the entries of one work item (up to 16
+// depth tiles that share the same 32 sorted columns, so their B rows stay hot
+// in L1/L2), keeps a register accumulator per half-wave and issues the C
+// atomics only when the row changes: a format-0 tile (all 32 rows equal)
+// costs EPS row flushes instead of 32.
+//
+// Column-major B/C (the reference's Swift_GPU layout).  Lanes run along the
+// 32 entries of a tile (two tiles per wave), each k reads 32 consecutive
+// sorted columns of B (coalesced), and equal-row runs inside the tile are
+// summed with a 5-step segmented shuffle reduction before one atomic per run
+// head (format 0/2 tiles); format 1 tiles skip the reduction.
+#include "swift_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+namespace swift {
+
+// ---------------------------------------------------------- type helpers --
+
+template <class T> struct Vec16;  // 16-byte vector of T
+template <> struct Vec16<float> { using type = float4; static constexpr int N = 4; };
+template <> struct Vec16<double> { using type = double2; static constexpr int N = 2; };
+template <> struct Vec16<__hip_bfloat16> { using type = uint4; static constexpr int N = 8; };
+template <> struct Vec16<__half> { using type = uint4; static constexpr int N = 8; };
+
+__device__ __forceinline__ float to_acc(float x) { return x; }
+__device__ __forceinline__ double to_acc(double x) { return x; }
+__device__ __forceinline__ float to_acc(__hip_bfloat16 x) { return __bfloat162float(x); }
+__device__ __forceinline__ float to_acc(__half x) { return __half2float(x); }
+
+template <class TA> struct AccOf { using type = float; };
+template <> struct AccOf<double> { using type = double; };
+
+// Load VEC consecutive elements of B starting at p (16-byte aligned when
+// VEC > 1) and widen them to the accumulator type.
+template <class TB, int VEC, class TC>
+__device__ __forceinline__ void load_b(const TB *__restrict__ p, TC (&o)[VEC]) {
+    if constexpr (VEC == 1) {
+        o[0] = to_acc(p[0]);
+    } else if constexpr (std::is_same<TB, float>::value) {
+        float4 v = *reinterpret_cast<const float4 *>(p);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    } else if constexpr (std::is_same<TB, double>::value) {
+        double2 v = *reinterpret_cast<const double2 *>(p);
+        o[0] = v.x; o[1] = v.y;
+    } else if constexpr (std::is_same<TB, __hip_bfloat16>::value) {
+        uint4 v = *reinterpret_cast<const uint4 *>(p);
+        unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            o[2 * i] = __uint_as_float(w[i] << 16);
+            o[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+        }
+    } else {  // __half
+        uint4 v = *reinterpret_cast<const uint4 *>(p);
+        unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            __half2 h = *reinterpret_cast<__half2 *>(&w[i]);
+            o[2 * i] = __low2float(h);
+            o[2 * i + 1] = __high2float(h);
+        }
+    }
+}
+
+__device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------- row-major kernels --
+
+struct RowArgs {
+    int nwork;
+    const int2 *work;           // regular: (first tile, tile count)
+    const int *tile_colidx;
+    const int *srow;            // sortrowidx
+    const void *sval;           // sortval (TV)
+    // reside
+    int nblk;                   // balance blocks (or reside columns)
+    const unsigned *colorblk;   // nullptr => unbalanced, one block per column
+    const int *blkstart, *blkstop;
+    const int *rptr, *rrow;
+    const void *rval;
+    int n;                      // columns of A
+    const int *perm;            // nullptr when B is already in sorted order
+    const void *B;
+    void *C;
+    int ldb, ldc;               // row strides (elements)
+    int kbase, kcount;          // K slice handled by this launch
+    const float *kscale;        // F16: per-k output scale (nullptr otherwise)
+    float gscale;               // F16: A-value scale
+};
+
+template <class TC, int VEC, int Q, int LPE, bool MASK>
+__device__ __forceinline__ void flush_row(const RowArgs &a, int row, int sub,
+                                          TC (&acc)[Q][VEC]) {
+    TC *crow = static_cast<TC *>(a.C) + (size_t)row * a.ldc + a.kbase;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+#pragma unroll
+        for (int v = 0; v < VEC; v++) {
+            int k = q * LPE * VEC + sub * VEC + v;
+            if (!MASK || k < a.kcount) {
+                TC x = acc[q][v];
+                if (a.kscale) x *= (TC)(a.kscale[a.kbase + k] * a.gscale);
+                else if (a.gscale != 1.0f) x *= (TC)a.gscale;
+                atomic_add(crow + k, x);
+            }
+            acc[q][v] = (TC)0;
+        }
+    }
+}
+
+template <class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+__device__ __forceinline__ void fma_row(const RowArgs &a, int bcol, int sub, TC val,
+                                        TC (&acc)[Q][VEC]) {
+    const TB *brow = static_cast<const TB *>(a.B) + (size_t)bcol * a.ldb + a.kbase;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        int k0 = q * LPE * VEC + sub * VEC;
+        TC b[VEC];
+        if (!MASK || k0 < a.kcount) {
+            load_b<TB, VEC, TC>(brow + k0, b);
+        } else {
+#pragma unroll
+            for (int v = 0; v < VEC; v++) b[v] = (TC)0;
+        }
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[q][v] += val * b[v];
+    }
+}
+
+// One wave per work item of up to `depth` tiles of one column block.
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+__global__ __launch_bounds__(256) void k_regular_row(RowArgs a) {
+    constexpr int EPS = 64 / LPE;       // entries in flight per wave
+    constexpr int STEPS = TILE / EPS;   // steps per tile
+    const int lane = threadIdx.x & 63;
+    const int h = lane / LPE, sub = lane % LPE;
+    const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (w >= a.nwork) return;
+    const int2 wk = a.work[w];
+    const int c0 = a.tile_colidx[wk.x];
+    const int my = lane & 31;
+    const int mycol = a.perm ? a.perm[c0 + my] : c0 + my;
+    const TV *sval = static_cast<const TV *>(a.sval);
+    TC acc[Q][VEC];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+    int cur = -1;
+    for (int t = wk.x; t < wk.x + wk.y; t++) {
+        const size_t e0 = (size_t)t * TILE;
+        const int myrow = a.srow[e0 + my];
+        const TC myval = to_acc(sval[e0 + my]);
+#pragma unroll
+        for (int i = 0; i < STEPS; i++) {
+            const int l = h + i * EPS;
+            const int r = __shfl(myrow, l, 64);
+            const TC v = __shfl(myval, l, 64);
+            const int bc = __shfl(mycol, l, 64);
+            if (r != cur) {
+                if (cur >= 0) flush_row<TC, VEC, Q, LPE, MASK>(a, cur, sub, acc);
+                cur = r;
+            }
+            fma_row<TB, TC, VEC, Q, LPE, MASK>(a, bc, sub, v, acc);
+        }
+    }
+    if (cur >= 0) flush_row<TC, VEC, Q, LPE, MASK>(a, cur, sub, acc);
+}
+
+// One wave per balance block (or per reside column when unbalanced): the
+// block's B row is loaded once, each entry is one row flush.
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+__global__ __launch_bounds__(256) void k_reside_row(RowArgs a) {
+    constexpr int EPS = 64 / LPE;
+    const int lane = threadIdx.x & 63;
+    const int h = lane / LPE, sub = lane % LPE;
+    const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (blk >= a.nblk) return;
+    int r, start, stop;
+    if (a.colorblk) {
+        const unsigned u = a.colorblk[blk];
+        r = (int)(u & 0x7FFFFFFFu);
+        const bool split = (u >> 31) != 0;
+        start = split ? a.blkstart[blk] : a.rptr[r];
+        stop = split ? a.blkstop[blk] : a.rptr[r + 1];
+    } else {
+        r = blk;
+        start = a.rptr[r];
+        stop = a.rptr[r + 1];
+    }
+    const int jj = a.n - 1 - r;
+    const int bcol = a.perm ? a.perm[jj] : jj;
+    const TB *brow = static_cast<const TB *>(a.B) + (size_t)bcol * a.ldb + a.kbase;
+    TC b[Q][VEC];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        int k0 = q * LPE * VEC + sub * VEC;
+        if (!MASK || k0 < a.kcount) load_b<TB, VEC, TC>(brow + k0, b[q]);
+        else
+#pragma unroll
+            for (int v = 0; v < VEC; v++) b[q][v] = (TC)0;
+    }
+    const TV *rval = static_cast<const TV *>(a.rval);
+    for (int e = start + h; e < stop; e += EPS) {
+        const int row = a.rrow[e];
+        const TC val = to_acc(rval[e]);
+        TC acc[Q][VEC];
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+            for (int v = 0; v < VEC; v++) acc[q][v] = val * b[q][v];
+        flush_row<TC, VEC, Q, LPE, MASK>(a, row, sub, acc);
+    }
+}
+
+// ---------------------------------------------------- col-major kernels --
+
+struct ColArgs {
+    int tilenum;
+    const int *tile_colidx, *tile_format;
+    const int *srow;
+    const void *sval;
+    int nblk;
+    const unsigned *colorblk;
+    const int *blkstart, *blkstop, *rptr, *rrow;
+    const void *rval;
+    int m, n;
+    const int *perm;
+    const void *B;              // K x n, B[k*ldb + col]
+    void *C;                    // K x m, C[k*ldc + row]
+    int ldb, ldc;
+    int kbase, kchunk, K;       // k range of blockIdx.y
+    const float *kscale;
+    float gscale;
+    // row-sorted groups (Swift_GPU_4)
+    const int *val_position, *rowsorted;
+};
+
+template <class TC>
+__device__ __forceinline__ TC kscale_of(const ColArgs &a, int k) {
+    return a.kscale ? (TC)(a.kscale[k] * a.gscale) : (TC)a.gscale;
+}
+
+// Two tiles per wave; blockIdx.y selects a K chunk.
+template <class TV, class TB, class TC>
+__global__ __launch_bounds__(256) void k_regular_col(ColArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int l = lane & 31;
+    const int t = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 5);
+    const bool valid = t < a.tilenum;
+    const size_t e = (size_t)(valid ? t : 0) * TILE + l;
+    const int row = valid ? a.srow[e] : 0;
+    const TC val = valid ? to_acc(static_cast<const TV *>(a.sval)[e]) : (TC)0;
+    const int c = valid ? a.tile_colidx[t] + l : 0;
+    const int bcol = a.perm ? a.perm[c] : c;
+    const int fmt = valid ? a.tile_format[t] : 1;
+    // equal-row runs inside the half: heads and each lane's run end
+    const int prev = __shfl_up(row, 1, 64);
+    const bool head = (l == 0) || prev != row;
+    const unsigned heads = (unsigned)(__ballot(head) >> (lane & 32));
+    const unsigned after = (l == 31) ? 0u : (heads >> (l + 1));
+    const int last = after ? l + __builtin_ctz(after) : 31;  // last lane of my run
+    const bool reduce = fmt != 1;
+    const int k0 = a.kbase + blockIdx.y * a.kchunk;
+    const int k1 = min(a.K, k0 + a.kchunk);
+    const TB *B = static_cast<const TB *>(a.B);
+    TC *C = static_cast<TC *>(a.C);
+    for (int k = k0; k < k1; k++) {
+        TC p = val * to_acc(B[(size_t)k * a.ldb + bcol]) * kscale_of<TC>(a, k);
+        if (reduce) {  // uniform per half-wave
+#pragma unroll
+            for (int off = 1; off < 32; off <<= 1) {
+                TC q = __shfl_down(p, off, 32);
+                if (l + off <= last) p += q;
+            }
+            if (valid && head) atomic_add(C + (size_t)k * a.ldc + row, p);
+        } else if (valid) {
+            atomic_add(C + (size_t)k * a.ldc + row, p);
+        }
+    }
+}
+
+// Balance block per half-wave; lanes along entries (<= 32 when balanced,
+// a whole reside column when unbalanced).
+template <class TV, class TB, class TC>
+__global__ __launch_bounds__(256) void k_reside_col(ColArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int l = lane & 31;
+    const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 5);
+    if (blk >= a.nblk) return;
+    int r, start, stop;
+    if (a.colorblk) {
+        const unsigned u = a.colorblk[blk];
+        r = (int)(u & 0x7FFFFFFFu);
+        const bool split = (u >> 31) != 0;
+        start = split ? a.blkstart[blk] : a.rptr[r];
+        stop = split ? a.blkstop[blk] : a.rptr[r + 1];
+    } else {
+        r = blk;
+        start = a.rptr[r];
+        stop = a.rptr[r + 1];
+    }
+    const int jj = a.n - 1 - r;
+    const int bcol = a.perm ? a.perm[jj] : jj;
+    const int k0 = a.kbase + blockIdx.y * a.kchunk;
+    const int k1 = min(a.K, k0 + a.kchunk);
+    const TB *B = static_cast<const TB *>(a.B);
+    TC *C = static_cast<TC *>(a.C);
+    const TV *rval = static_cast<const TV *>(a.rval);
+    for (int e = start + l; e < stop; e += 32) {
+        const int row = a.rrow[e];
+        const TC val = to_acc(rval[e]);
+        for (int k = k0; k < k1; k++)
+            atomic_add(C + (size_t)k * a.ldc + row,
+                       val * to_acc(B[(size_t)k * a.ldb + bcol]) * kscale_of<TC>(a, k));
+    }
+}
+
+// Row-sorted groups of 8 tiles (256 entries, one per thread).  Products go to
+// their sorted slot in LDS; each run head of equal rows sums its run and
+// issues one atomic per (run, k).  Run bounds come from the sorted keys.
+template <class TV, class TB, class TC>
+__global__ __launch_bounds__(256) void k_regular_col_rowsort(ColArgs a) {
+    __shared__ TC prod[256];
+    const int g = blockIdx.x;
+    const int t0 = g * 8;
+    const int nt = min(8, a.tilenum - t0);
+    const int len = nt * TILE;
+    const int i = threadIdx.x;
+    const size_t base = (size_t)t0 * TILE;
+    const bool valid = i < len;
+    TC val = (TC)0;
+    int bcol = 0, pos = 0, srow = 0, runlen = 0;
+    if (valid) {
+        const int t = t0 + (i >> 5);
+        val = to_acc(static_cast<const TV *>(a.sval)[base + i]);
+        const int c = a.tile_colidx[t] + (i & 31);
+        bcol = a.perm ? a.perm[c] : c;
+        pos = a.val_position[base + i];
+        srow = a.rowsorted[base + i];
+        const bool head = i == 0 || a.rowsorted[base + i - 1] != srow;
+        if (head) {
+            int e = i + 1;
+            while (e < len && a.rowsorted[base + e] == srow) e++;
+            runlen = e - i;
+        }
+    }
+    const int k0 = a.kbase + blockIdx.y * a.kchunk;
+    const int k1 = min(a.K, k0 + a.kchunk);
+    const TB *B = static_cast<const TB *>(a.B);
+    TC *C = static_cast<TC *>(a.C);
+    for (int k = k0; k < k1; k++) {
+        if (valid) prod[pos] = val * to_acc(B[(size_t)k * a.ldb + bcol]);
+        __syncthreads();
+        if (runlen > 0) {
+            TC s = (TC)0;
+            for (int j = 0; j < runlen; j++) s += prod[i + j];
+            atomic_add(C + (size_t)k * a.ldc + srow, s * kscale_of<TC>(a, k));
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- launch --
+
+// Optional events between the phases of one call (zero | regular | reside).
+struct PhaseEvents {
+    hipEvent_t ev[4];
+};
+static thread_local PhaseEvents *g_phase = nullptr;
+
+static inline void mark(int i, hipStream_t s) {
+    if (g_phase) (void)hipEventRecord(g_phase->ev[i], s);
+}
+
+
+struct RowTarget {
+    int vec, q, lpe;
+    bool mask;
+};
+
+// Splits [0, K) into slices handled by one kernel each: the largest
+// 16-byte-vector shape that fits, then masked scalar slices of <= 64.
+static std::vector<std::pair<int, RowTarget>> plan_row_slices(int K, int vec, bool aligned) {
+    std::vector<std::pair<int, RowTarget>> out;
+    int k = 0;
+    if (aligned) {
+        static const int qs[] = {4, 2, 1};
+        while (K - k >= vec * 4) {
+            int rem = K - k;
+            bool done = false;
+            for (int q : qs) {
+                if (rem >= vec * 64 * q) {
+                    out.push_back({k, {vec, q, 64, false}});
+                    k += vec * 64 * q;
+                    done = true;
+                    break;
+                }
+            }
+            if (done) continue;
+            for (int lpe = 32; lpe >= 4; lpe >>= 1) {
+                if (lpe == 4 && vec != 8) break;
+                if (rem >= vec * lpe) {
+                    out.push_back({k, {vec, 1, lpe, false}});
+                    k += vec * lpe;
+                    done = true;
+                    break;
+                }
+            }
+            if (!done) break;
+        }
+    }
+    while (k < K) {
+        out.push_back({k, {1, 1, 64, true}});
+        k += std::min(64, K - k);
+    }
+    return out;
+}
+
+template <class TV, class TB, class TC, int VEC>
+static int launch_row_vec(const RowArgs &a0, const RowTarget &t, bool regular, hipStream_t s) {
+    const unsigned nitems = regular ? (unsigned)a0.nwork : (unsigned)a0.nblk;
+    if (nitems == 0) return SWIFT_OK;
+    dim3 grid(ceil_div((size_t)nitems * 64, 256)), block(256);
+#define SWIFT_ROW_CASE(Q_, L_)                                                        \
+    if (t.q == Q_ && t.lpe == L_) {                                                   \
+        if (regular) hipLaunchKernelGGL((k_regular_row<TV, TB, TC, VEC, Q_, L_, false>), grid, block, 0, s, a0); \
+        else hipLaunchKernelGGL((k_reside_row<TV, TB, TC, VEC, Q_, L_, false>), grid, block, 0, s, a0); \
+        SWIFT_CHECK_LAUNCH();                                                         \
+        return SWIFT_OK;                                                              \
+    }
+    SWIFT_ROW_CASE(4, 64)
+    SWIFT_ROW_CASE(2, 64)
+    SWIFT_ROW_CASE(1, 64)
+    SWIFT_ROW_CASE(1, 32)
+    SWIFT_ROW_CASE(1, 16)
+    SWIFT_ROW_CASE(1, 8)
+    if constexpr (VEC == 8) { SWIFT_ROW_CASE(1, 4) }
+#undef SWIFT_ROW_CASE
+    set_error("no row kernel for vec=%d q=%d lpe=%d", t.vec, t.q, t.lpe);
+    return SWIFT_ERR_UNSUPPORTED;
+}
+
+template <class TV, class TB, class TC>
+static int launch_row(const RowArgs &a0, const RowTarget &t, bool regular, hipStream_t s) {
+    constexpr int V = Vec16<TB>::N;
+    if (t.mask) {
+        const unsigned nitems = regular ? (unsigned)a0.nwork : (unsigned)a0.nblk;
+        if (nitems == 0) return SWIFT_OK;
+        dim3 grid(ceil_div((size_t)nitems * 64, 256)), block(256);
+        if (regular) hipLaunchKernelGGL((k_regular_row<TV, TB, TC, 1, 1, 64, true>), grid, block, 0, s, a0);
+        else hipLaunchKernelGGL((k_reside_row<TV, TB, TC, 1, 1, 64, true>), grid, block, 0, s, a0);
+        SWIFT_CHECK_LAUNCH();
+        return SWIFT_OK;
+    }
+    return launch_row_vec<TV, TB, TC, V>(a0, t, regular, s);
+}
+
+template <class TV, class TB, class TC>
+static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned flags,
+                    const void *B, void *C, const void *sval, const void *rval,
+                    const float *kscale, float gscale, hipStream_t s) {
+    RowArgs a{};
+    a.nwork = M->nwork;
+    a.work = M->work;
+    a.tile_colidx = M->tile_colidx;
+    a.srow = M->sortrowidx;
+    a.sval = sval;
+    const bool unbalanced = kernel == SWIFT_KERNEL_UNBALANCED;
+    a.nblk = unbalanced ? M->reside_col : M->numblk;
+    a.colorblk = unbalanced ? nullptr : M->colorblk;
+    a.blkstart = M->blkstart;
+    a.blkstop = M->blkstop;
+    a.rptr = M->reside_cscptr;
+    a.rrow = M->reside_cscrowidx;
+    a.rval = rval;
+    a.n = M->n;
+    a.perm = (flags & SWIFT_B_SORTED) ? nullptr : M->perm;
+    a.B = B;
+    a.C = C;
+    a.ldb = K;
+    a.ldc = K;
+    a.kscale = kscale;
+    a.gscale = gscale;
+    constexpr int V = Vec16<TB>::N;
+    const bool aligned = (K % V) == 0 && ((uintptr_t)B % 16) == 0;
+    const auto slices = plan_row_slices(K, V, aligned);
+    for (auto &sl : slices) {
+        a.kbase = sl.first;
+        const RowTarget &t = sl.second;
+        a.kcount = t.mask ? std::min(64, K - sl.first) : t.vec * t.lpe * t.q;
+        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, true, s)));
+    }
+    mark(2, s);
+    for (auto &sl : slices) {
+        a.kbase = sl.first;
+        const RowTarget &t = sl.second;
+        a.kcount = t.mask ? std::min(64, K - sl.first) : t.vec * t.lpe * t.q;
+        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, false, s)));
+    }
+    mark(3, s);
+    return SWIFT_OK;
+}
+
+template <class TV, class TB, class TC>
+static int spmm_col(const swift_matrix *M, int K, swift_kernel kernel, unsigned flags,
+                    const void *B, void *C, const void *sval, const void *rval,
+                    const float *kscale, float gscale, hipStream_t s) {
+    ColArgs a{};
+    a.tilenum = M->tilenum;
+    a.tile_colidx = M->tile_colidx;
+    a.tile_format = M->tile_format;
+    a.srow = M->sortrowidx;
+    a.sval = sval;
+    const bool unbalanced = kernel == SWIFT_KERNEL_UNBALANCED;
+    a.nblk = unbalanced ? M->reside_col : M->numblk;
+    a.colorblk = unbalanced ? nullptr : M->colorblk;
+    a.blkstart = M->blkstart;
+    a.blkstop = M->blkstop;
+    a.rptr = M->reside_cscptr;
+    a.rrow = M->reside_cscrowidx;
+    a.rval = rval;
+    a.m = M->m;
+    a.n = M->n;
+    a.perm = (flags & SWIFT_B_SORTED) ? nullptr : M->perm;
+    a.B = B;
+    a.C = C;
+    a.ldb = M->n;
+    a.ldc = M->m;
+    a.kbase = 0;
+    a.K = K;
+    a.kscale = kscale;
+    a.gscale = gscale;
+    a.val_position = M->val_position;
+    a.rowsorted = M->rowsorted;
+    // K chunk per blockIdx.y: the reference's Swift_GPU_3/4 use 32-wide chunks
+    // (src/Swift_3.h:338); the others walk all of K in one wave.
+    const bool chunked = kernel == SWIFT_KERNEL_KCHUNK || kernel == SWIFT_KERNEL_ROWSORT ||
+                         kernel == SWIFT_KERNEL_DEFAULT;
+    a.kchunk = chunked ? std::min(K, 32) : K;
+    const unsigned ky = (unsigned)((K + a.kchunk - 1) / a.kchunk);
+    if (kernel == SWIFT_KERNEL_ROWSORT) {
+        if (!M->has_rowsort) {
+            set_error("SWIFT_KERNEL_ROWSORT needs swift_row_idx_sort first");
+            return SWIFT_ERR_ARG;
+        }
+        unsigned groups = (unsigned)((M->tilenum + 7) / 8);
+        if (groups) {
+            hipLaunchKernelGGL((k_regular_col_rowsort<TV, TB, TC>), dim3(groups, ky), dim3(256), 0,
+                               s, a);
+            SWIFT_CHECK_LAUNCH();
+        }
+    } else if (M->tilenum) {
+        hipLaunchKernelGGL((k_regular_col<TV, TB, TC>),
+                           dim3(ceil_div((size_t)M->tilenum * 32, 256), ky), dim3(256), 0, s, a);
+        SWIFT_CHECK_LAUNCH();
+    }
+    mark(2, s);
+    if (a.nblk) {
+        hipLaunchKernelGGL((k_reside_col<TV, TB, TC>),
+                           dim3(ceil_div((size_t)a.nblk * 32, 256), ky), dim3(256), 0, s, a);
+        SWIFT_CHECK_LAUNCH();
+    }
+    mark(3, s);
+    return SWIFT_OK;
+}
+
+int spmm_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dtype,
+              swift_kernel kernel, unsigned flags, const void *B, void *C,
+              const float *kscale, hipStream_t s);
+
+int spmm_timed_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dtype,
+                    swift_kernel kernel, unsigned flags, const void *B, void *C,
+                    const float *kscale, hipStream_t s, float *ms3) {
+    PhaseEvents pe;
+    for (auto &e : pe.ev) SWIFT_HIP(hipEventCreate(&e));
+    g_phase = &pe;
+    int rc = spmm_impl(M, K, layout, dtype, kernel, flags, B, C, kscale, s);
+    g_phase = nullptr;
+    if (rc == SWIFT_OK) {
+        SWIFT_HIP(hipEventSynchronize(pe.ev[3]));
+        for (int i = 0; i < 3; i++) SWIFT_HIP(hipEventElapsedTime(&ms3[i], pe.ev[i], pe.ev[i + 1]));
+    }
+    for (auto &e : pe.ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int spmm_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dtype,
+              swift_kernel kernel, unsigned flags, const void *B, void *C,
+              const float *kscale, hipStream_t s) {
+    if (!M || K <= 0 || !B || !C) {
+        set_error("swift_spmm: bad arguments");
+        return SWIFT_ERR_ARG;
+    }
+    if (layout != SWIFT_ROW_MAJOR && layout != SWIFT_COL_MAJOR) {
+        set_error("swift_spmm: bad layout");
+        return SWIFT_ERR_ARG;
+    }
+    const bool lowp = dtype == SWIFT_BF16 || dtype == SWIFT_F16;
+    if (!lowp && dtype != M->vtype) {
+        set_error("swift_spmm: dtype %d differs from the build value type %d", (int)dtype,
+                  (int)M->vtype);
+        return SWIFT_ERR_ARG;
+    }
+    if (lowp && (M->lp_type != dtype || !M->sortval_lp)) {
+        set_error("swift_spmm: call swift_prepare_values(%d) first", (int)dtype);
+        return SWIFT_ERR_ARG;
+    }
+    if (kernel == SWIFT_KERNEL_ROWSORT && layout == SWIFT_ROW_MAJOR) {
+        set_error("SWIFT_KERNEL_ROWSORT is a column-major kernel");
+        return SWIFT_ERR_UNSUPPORTED;
+    }
+    const size_t csize = (dtype == SWIFT_F64) ? 8 : 4;
+    mark(0, s);
+    if (!(flags & SWIFT_ACCUMULATE))
+        SWIFT_TRY(fill_bytes_zero(C, (size_t)M->m * K * csize, s));
+    mark(1, s);
+    const void *sv = lowp ? M->sortval_lp : M->sortval;
+    const void *rv = lowp ? M->reside_val_lp : M->reside_val;
+    const float gs = M->lp_scale;
+    if (layout == SWIFT_ROW_MAJOR) {
+        switch (dtype) {
+            case SWIFT_F32: return spmm_row<float, float, float>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
+            case SWIFT_F64: return spmm_row<double, double, double>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
+            case SWIFT_BF16: return spmm_row<__hip_bfloat16, __hip_bfloat16, float>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
+            case SWIFT_F16: return spmm_row<__half, __half, float>(M, K, kernel, flags, B, C, sv, rv, kscale, gs, s);
+        }
+    } else {  // the row-sorted kernel applies the scale in its run sums
+        switch (dtype) {
+            case SWIFT_F32: return spmm_col<float, float, float>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
+            case SWIFT_F64: return spmm_col<double, double, double>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
+            case SWIFT_BF16: return spmm_col<__hip_bfloat16, __hip_bfloat16, float>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
+            case SWIFT_F16: return spmm_col<__half, __half, float>(M, K, kernel, flags, B, C, sv, rv, kscale, gs, s);
+        }
+    }
+    set_error("swift_spmm: unsupported dtype");
+    return SWIFT_ERR_UNSUPPORTED;
+}
+
+// ------------------------------------------------ low-precision storage --
+
+template <class TO>
+__device__ __forceinline__ TO from_f32(float x);
+template <> __device__ __forceinline__ __hip_bfloat16 from_f32(float x) { return __float2bfloat16(x); }
+template <> __device__ __forceinline__ __half from_f32(float x) { return __float2half(x); }
+
+template <class TI, class TO>
+__global__ void k_convert(const TI *__restrict__ in, TO *__restrict__ out, size_t n, float inv) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = from_f32<TO>((float)in[i] * inv);
+}
+
+// Per-k power-of-two scale so that max |B[:,k]| / scale <= 2^14 (F16 max is
+// 65504): one atomicMax on the float bits per block and column.
+__global__ void k_colmax(const float *__restrict__ in, size_t rows, int K, int col_major,
+                         unsigned *__restrict__ cmax) {
+    size_t total = rows * (size_t)K;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < total; i += stride) {
+        int k = col_major ? (int)(i / rows) : (int)(i % K);
+        float v = fabsf(in[i]);
+        atomicMax(&cmax[k], __float_as_uint(v));
+    }
+}
+
+__global__ void k_scale_from_max(const unsigned *__restrict__ cmax, int K, float *__restrict__ sc) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    float mx = __uint_as_float(cmax[k]);
+    float s = 1.0f;
+    if (mx > 0.f) {
+        int e;
+        frexpf(mx / 16384.0f, &e);
+        s = ldexpf(1.0f, e);
+    }
+    sc[k] = s;
+}
+
+template <class TO>
+__global__ void k_convert_scaled(const float *__restrict__ in, TO *__restrict__ out, size_t rows,
+                                 int K, int col_major, const float *__restrict__ sc) {
+    size_t total = rows * (size_t)K;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < total; i += stride) {
+        int k = col_major ? (int)(i / rows) : (int)(i % K);
+        out[i] = from_f32<TO>(in[i] / sc[k]);
+    }
+}
+
+static unsigned grid_cap(size_t n) {
+    size_t g = (n + 255) / 256;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 65536));
+}
+
+int convert_dense_impl(const float *in, void *out, size_t rows, int K, swift_layout layout,
+                       swift_dtype dtype, float *col_scale, hipStream_t s) {
+    size_t n = rows * (size_t)K;
+    if (n == 0) return SWIFT_OK;
+    if (dtype == SWIFT_BF16) {
+        hipLaunchKernelGGL((k_convert<float, __hip_bfloat16>), dim3(grid_cap(n)), dim3(256), 0, s,
+                           in, (__hip_bfloat16 *)out, n, 1.0f);
+    } else if (dtype == SWIFT_F16) {
+        if (col_scale) {
+            Buffer cmax;
+            SWIFT_TRY(cmax.alloc(sizeof(unsigned) * (size_t)K));
+            SWIFT_TRY(fill_i32(cmax.as<int>(), 0, (size_t)K, s));
+            hipLaunchKernelGGL(k_colmax, dim3(grid_cap(n)), dim3(256), 0, s, in, rows, K,
+                               layout == SWIFT_COL_MAJOR, cmax.as<unsigned>());
+            hipLaunchKernelGGL(k_scale_from_max, dim3(ceil_div(K, 256)), dim3(256), 0, s,
+                               cmax.as<unsigned>(), K, col_scale);
+            hipLaunchKernelGGL((k_convert_scaled<__half>), dim3(grid_cap(n)), dim3(256), 0, s, in,
+                               (__half *)out, rows, K, layout == SWIFT_COL_MAJOR, col_scale);
+            SWIFT_CHECK_LAUNCH();
+            SWIFT_HIP(hipStreamSynchronize(s));
+            return SWIFT_OK;
+        }
+        hipLaunchKernelGGL((k_convert<float, __half>), dim3(grid_cap(n)), dim3(256), 0, s, in,
+                           (__half *)out, n, 1.0f);
+    } else {
+        set_error("swift_convert_dense: dtype must be BF16 or F16");
+        return SWIFT_ERR_ARG;
+    }
+    SWIFT_CHECK_LAUNCH();
+    return SWIFT_OK;
+}
+
+__global__ void k_absmax_f32(const float *__restrict__ a, size_t n, unsigned *out) {
+    unsigned v = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        v = max(v, __float_as_uint(fabsf(a[i])));
+    atomicMax(out, v);
+}
+
+__global__ void k_absmax_f64(const double *__restrict__ a, size_t n, unsigned *out) {
+    unsigned v = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        v = max(v, __float_as_uint(fabsf((float)a[i])));
+    atomicMax(out, v);
+}
+
+// A's values to BF16, or to F16 divided by one power-of-two scale chosen so
+// that max |a| / scale <= 2^14.
+int prepare_values_impl(swift_matrix *M, swift_dtype dtype, hipStream_t s) {
+    if (dtype != SWIFT_BF16 && dtype != SWIFT_F16) {
+        set_error("swift_prepare_values: dtype must be BF16 or F16");
+        return SWIFT_ERR_ARG;
+    }
+    if (M->sortval_lp) { (void)hipFree(M->sortval_lp); M->sortval_lp = nullptr; }
+    if (M->reside_val_lp) { (void)hipFree(M->reside_val_lp); M->reside_val_lp = nullptr; }
+    SWIFT_HIP(hipMalloc(&M->sortval_lp, std::max<size_t>(2, (size_t)M->regular_nnz * 2)));
+    SWIFT_HIP(hipMalloc(&M->reside_val_lp, std::max<size_t>(2, (size_t)M->reside_nnz * 2)));
+    float scale = 1.0f;
+    if (dtype == SWIFT_F16) {
+        Buffer mx;
+        SWIFT_TRY(mx.alloc(sizeof(unsigned)));
+        SWIFT_TRY(fill_i32(mx.as<int>(), 0, 1, s));
+        if (M->vtype == SWIFT_F64) {
+            if (M->regular_nnz) hipLaunchKernelGGL(k_absmax_f64, dim3(grid_cap(M->regular_nnz)), dim3(256), 0, s, (const double *)M->sortval, (size_t)M->regular_nnz, mx.as<unsigned>());
+            if (M->reside_nnz) hipLaunchKernelGGL(k_absmax_f64, dim3(grid_cap(M->reside_nnz)), dim3(256), 0, s, (const double *)M->reside_val, (size_t)M->reside_nnz, mx.as<unsigned>());
+        } else {
+            if (M->regular_nnz) hipLaunchKernelGGL(k_absmax_f32, dim3(grid_cap(M->regular_nnz)), dim3(256), 0, s, (const float *)M->sortval, (size_t)M->regular_nnz, mx.as<unsigned>());
+            if (M->reside_nnz) hipLaunchKernelGGL(k_absmax_f32, dim3(grid_cap(M->reside_nnz)), dim3(256), 0, s, (const float *)M->reside_val, (size_t)M->reside_nnz, mx.as<unsigned>());
+        }
+        SWIFT_CHECK_LAUNCH();
+        unsigned bits = 0;
+        SWIFT_HIP(hipMemcpyAsync(&bits, mx.ptr, 4, hipMemcpyDeviceToHost, s));
+        SWIFT_HIP(hipStreamSynchronize(s));
+        float amax;
+        std::memcpy(&amax, &bits, 4);
+        if (amax > 0.f) {
+            int e;
+            std::frexp(amax / 16384.0f, &e);
+            scale = std::ldexp(1.0f, e);
+        }
+    }
+    const float inv = 1.0f / scale;
+    auto conv = [&](const void *src, void *dst, size_t n) -> int {
+        if (n == 0) return SWIFT_OK;
+        if (M->vtype == SWIFT_F64) {
+            if (dtype == SWIFT_BF16) hipLaunchKernelGGL((k_convert<double, __hip_bfloat16>), dim3(grid_cap(n)), dim3(256), 0, s, (const double *)src, (__hip_bfloat16 *)dst, n, inv);
+            else hipLaunchKernelGGL((k_convert<double, __half>), dim3(grid_cap(n)), dim3(256), 0, s, (const double *)src, (__half *)dst, n, inv);
+        } else {
+            if (dtype == SWIFT_BF16) hipLaunchKernelGGL((k_convert<float, __hip_bfloat16>), dim3(grid_cap(n)), dim3(256), 0, s, (const float *)src, (__hip_bfloat16 *)dst, n, inv);
+            else hipLaunchKernelGGL((k_convert<float, __half>), dim3(grid_cap(n)), dim3(256), 0, s, (const float *)src, (__half *)dst, n, inv);
+        }
+        SWIFT_CHECK_LAUNCH();
+        return SWIFT_OK;
+    };
+    SWIFT_TRY(conv(M->sortval, M->sortval_lp, (size_t)M->regular_nnz));
+    SWIFT_TRY(conv(M->reside_val, M->reside_val_lp, (size_t)M->reside_nnz));
+    SWIFT_HIP(hipStreamSynchronize(s));
+    M->lp_type = dtype;
+    M->lp_scale = scale;
+    return SWIFT_OK;
+}
+
+}  // namespace swift

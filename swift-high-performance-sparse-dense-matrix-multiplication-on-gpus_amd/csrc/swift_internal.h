@@ -1,0 +1,151 @@
+// swift_internal.h -- shared declarations of the Swift SpMM library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "swift_spmm.h"
+
+namespace swift {
+
+constexpr int TILE = 32;   // tile width (TILESIZE/slidesize of the reference)
+constexpr int WAVE = 64;   // CDNA wavefront
+
+// Last error text of the calling thread (swift_last_error()).
+void set_error(const char *fmt, ...);
+
+#define SWIFT_HIP(call)                                                       \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            ::swift::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,     \
+                               hipGetErrorString(e_));                        \
+            return SWIFT_ERR_HIP;                                             \
+        }                                                                     \
+    } while (0)
+
+#define SWIFT_TRY(call)                                                       \
+    do {                                                                      \
+        int s_ = (call);                                                      \
+        if (s_ != SWIFT_OK) return s_;                                        \
+    } while (0)
+
+#define SWIFT_CHECK_LAUNCH() SWIFT_HIP(hipGetLastError())
+
+inline size_t value_size(swift_dtype t) {
+    switch (t) {
+        case SWIFT_F32: return 4;
+        case SWIFT_F64: return 8;
+        case SWIFT_BF16: return 2;
+        case SWIFT_F16: return 2;
+    }
+    return 0;
+}
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Device workspace that grows on demand; owned by a matrix or a call site.
+struct Buffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int alloc(size_t n);   // (re)allocates only when n > bytes
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// ---------------------------------------------------------------- scan/sort
+
+// Exclusive scan of n ints; out may alias in.  If total != nullptr the sum is
+// written there (device pointer).  tmp needs scan_temp_bytes(n).
+size_t scan_temp_bytes(size_t n);
+int exclusive_scan(const int *in, int *out, size_t n, int *total, void *tmp,
+                   hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs on bits [0, bits).  If vals_in is
+// null the values are the input positions 0..n-1.  Results land in
+// keys_out/vals_out; the *_alt buffers (n each) are scratch.  tmp needs
+// radix_temp_bytes(n).
+size_t radix_temp_bytes(size_t n);
+int radix_sort_pairs(const unsigned *keys_in, const unsigned *vals_in,
+                     unsigned *keys_out, unsigned *vals_out, unsigned *keys_alt,
+                     unsigned *vals_alt, size_t n, int bits, void *tmp,
+                     hipStream_t s);
+
+int bits_for(unsigned max_value);
+
+// Fill helpers (kernels, never hipMemsetAsync, so that graph capture of the
+// SpMM stays valid).
+int fill_i32(int *p, int v, size_t n, hipStream_t s);
+int fill_bytes_zero(void *p, size_t bytes, hipStream_t s);
+int iota_i32(int *p, size_t n, hipStream_t s);
+
+}  // namespace swift
+
+// ------------------------------------------------------------------ matrix
+// Device-resident Swift format.  Field names follow the reference's
+// slide_matrix (src/format.h:6-39) so every index array can be dumped and
+// compared with the CPU restatement.
+struct swift_matrix {
+    int m = 0, n = 0, nnz = 0;
+    swift_dtype vtype = SWIFT_F32;  // value type of the build (F32 or F64)
+    int device = 0;
+
+    // scalars
+    int tilenum = 0, regular_nnz = 0, reside_col = 0, reside_nnz = 0;
+    int segsum = 0, numblk = 0, nsteps = 0, max_col_nnz = 0;
+    int has_rowsort = 0;
+
+    // column sort (ColSort): perm[i] = original column at sorted position i
+    int *perm = nullptr, *sortnnz = nullptr, *cptr = nullptr;  // cptr n+1
+    int *sortrowidx_tmp = nullptr;
+    void *sortval_tmp = nullptr;
+    int *sortsrc_tmp = nullptr;  // CSR position of each sorted-CSC entry
+
+    // staircase steps (our derived schedule data)
+    int *step_col = nullptr, *step_base = nullptr, *step_tall = nullptr;
+    int *step_tileoff = nullptr;  // nsteps+1
+
+    // regular tiles
+    int *tile_ptr = nullptr, *tile_len = nullptr, *tile_colidx = nullptr;
+    int *tile_tall = nullptr, *tile_format = nullptr;
+    int *sortrowidx = nullptr;
+    void *sortval = nullptr;
+    int *sortsrc = nullptr;          // CSR position of each regular entry
+
+    // categorisation
+    int *countrow = nullptr, *segmentoffset = nullptr, *sortrowindex = nullptr;
+
+    // reside (irregular) CSC over reversed sorted columns
+    int *reside_cscptr = nullptr, *reside_cscrowidx = nullptr;
+    void *reside_val = nullptr;
+    int *reside_src = nullptr;
+
+    // balance
+    unsigned *colorblk = nullptr;
+    int *blkstart = nullptr, *blkstop = nullptr, *blkptr = nullptr;
+
+    // row sort (Swift_GPU_4's row_idx_sort)
+    int *val_position = nullptr, *val_offset = nullptr, *rowsorted = nullptr;
+    int *rowsort_src = nullptr;      // position inside the group of sorted slot
+
+    // CSR copy of the input pattern (kept for the transposed layout / SDDMM)
+    int *csr_rowptr = nullptr, *csr_colidx = nullptr;
+
+    // low-precision value copies (swift_prepare_values)
+    void *sortval_lp = nullptr, *reside_val_lp = nullptr;
+    swift_dtype lp_type = SWIFT_F32;
+    float lp_scale = 1.0f;
+
+    // column-tile work list for the regular kernels: (tile start, count)
+    int nwork = 0;
+    int2 *work = nullptr;        // (first tile, tile count) per work item
+    int work_depth = 0;          // max tiles per work item
+
+    swift::Buffer scratch;       // reusable device workspace
+};

@@ -1,0 +1,179 @@
+"""16-bit storage forms and the backward kernels on the GPU.
+
+BF16 / F16 (per-column scaled) storage of A's values and of B with FP32
+accumulation: per-element error normalised by |A||B| <= 1e-2 against the FP64
+oracle (per-row max relative error is printed).  Backward: dB = A^T dC through
+the transposed Swift layout and through the fused kernel, dvals = SDDMM(dC, B),
+both against FP64 closed forms; the autograd wrapper against FP64 finite
+differences of the loss."""
+import numpy as np
+import pytest
+
+import gpu_common
+import matgen
+
+pytestmark = pytest.mark.gpu
+
+CASES = gpu_common.cases()
+
+
+def setup(swift, oracle, name, K, seed=0):
+    import torch
+    m, n, rp, ci, v = CASES[name]
+    vv = np.asarray(v, np.float32)
+    B = np.random.default_rng(seed).uniform(-1, 1, (n, K)).astype(np.float32)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=torch.float32)
+    return M, B, C64, absC
+
+
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "equalrows_100x70",
+                                  "narrow_50x20"])
+@pytest.mark.parametrize("K", [32, 128, 256])
+@pytest.mark.parametrize("layout", ["ROW", "COL"])
+def test_bf16_storage(swift, oracle, cuda, name, K, layout):
+    import torch
+    M, B, C64, absC = setup(swift, oracle, name, K)
+    lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
+    M.prepare_values(swift.BF16)
+    Bt = torch.tensor(B if lay == swift.ROW_MAJOR else B.T.copy(), device="cuda")
+    Bh = swift.convert_dense(Bt, swift.BF16, layout=lay)
+    C = M.spmm(Bh, layout=lay, dtype=swift.BF16).cpu().numpy()
+    if lay == swift.COL_MAJOR:
+        C = C.T
+    err = gpu_common.rel_err(C, C64, absC)
+    print("bf16", name, K, layout, "err", err,
+          "row_max_rel", float(np.max(gpu_common.row_max_rel(C, C64))), flush=True)
+    assert err <= 1e-2
+    M.close()
+
+
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100"])
+@pytest.mark.parametrize("K", [32, 128])
+@pytest.mark.parametrize("layout", ["ROW", "COL"])
+def test_f16_scaled_storage(swift, oracle, cuda, name, K, layout):
+    import torch
+    M, B, C64, absC = setup(swift, oracle, name, K)
+    lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
+    # widen the dynamic range so that the per-column scales matter
+    B = B * np.logspace(-3, 3, K, dtype=np.float32)[None, :]
+    C64, absC = oracle.spmm_f64(*CASES[name][:4], np.asarray(CASES[name][4], np.float32)
+                                .astype(np.float64), B.astype(np.float64))
+    M.prepare_values(swift.F16)
+    Bt = torch.tensor(B if lay == swift.ROW_MAJOR else B.T.copy(), device="cuda")
+    Bh, sc = swift.convert_dense(Bt, swift.F16, layout=lay, col_scale=True)
+    C = M.spmm_scaled(Bh, sc, layout=lay).cpu().numpy()
+    if lay == swift.COL_MAJOR:
+        C = C.T
+    err = gpu_common.rel_err(C, C64, absC)
+    print("f16", name, K, layout, "err", err, flush=True)
+    assert err <= 1e-2
+    M.close()
+
+
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "equalrows_100x70",
+                                  "unsorted_rows_64x96", "narrow_50x20"])
+@pytest.mark.parametrize("K", [32, 128])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_backward(swift, oracle, cuda, name, K, dt):
+    import torch
+    print("case", name, K, dt, flush=True)
+    m, n, rp, ci, v = CASES[name]
+    npdt = np.float32 if dt == "f32" else np.float64
+    dtype = torch.float32 if dt == "f32" else torch.float64
+    tol = 1e-5 if dt == "f32" else 1e-12
+    vv = np.asarray(v, npdt)
+    rng = np.random.default_rng(K)
+    B = rng.standard_normal((n, K)).astype(npdt)
+    dC = rng.standard_normal((m, K)).astype(npdt)
+    dB64 = oracle.spmm_t_f64(m, n, rp, ci, vv.astype(np.float64), dC.astype(np.float64))
+    absB = oracle.spmm_t_f64(m, n, rp, ci, np.abs(vv.astype(np.float64)),
+                             np.abs(dC.astype(np.float64)))
+    dv64 = oracle.sddmm_f64(m, n, rp, ci, dC.astype(np.float64), B.astype(np.float64))
+    absv = oracle.sddmm_f64(m, n, rp, ci, np.abs(dC.astype(np.float64)),
+                            np.abs(B.astype(np.float64)))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=dtype)
+    dCt = torch.tensor(dC, device="cuda")
+    Bt = torch.tensor(B, device="cuda")
+    dB_f, dv_f = M.backward_fused(dCt, Bt)
+    dB_t = M.transpose().spmm(dCt)
+    dv_s = M.sddmm(dCt, Bt)
+    for dB in (dB_f, dB_t):
+        assert gpu_common.rel_err(dB.cpu().numpy(), dB64, absB) <= tol
+    for dv in (dv_f, dv_s):
+        got = dv.cpu().numpy().astype(np.float64)
+        assert float(np.max(np.abs(got - dv64) / (absv + 1e-30), initial=0)) <= tol
+    M.close()
+
+
+def test_transpose_layout_bit_exact(swift, oracle, cuda):
+    """The transposed Swift layout equals the oracle's format of A^T."""
+    import scipy.sparse as sp
+    import torch
+    m, n, rp, ci, v = CASES["powerlaw_3000x2500"]
+    vv = np.asarray(v, np.float32)
+    M = swift.SwiftMatrix(m, n, rp, ci, vv)
+    T = M.transpose()
+    At = sp.csr_matrix((vv.astype(np.float64), ci, rp), shape=(m, n)).T.tocsr()
+    # CSR of A^T with rows in ascending column order = A's CSC (stable)
+    ptr, row, val, _ = oracle.csr2csc(m, n, rp, ci, vv.astype(np.float64))
+    ref = oracle.build(n, m, ptr, row, val)
+    got = T.arrays()
+    for k in ("perm", "sortnnz", "tile_colidx", "sortrowidx", "reside_cscptr",
+              "reside_cscrowidx", "tile_format", "colorblk"):
+        assert np.array_equal(got[k], np.asarray(ref[k]).astype(got[k].dtype)), k
+    assert np.array_equal(got["sortval"], ref["sortval"].astype(np.float32))
+    assert At.nnz == len(ci)
+    M.close()
+
+
+def test_update_values(swift, oracle, cuda):
+    import torch
+    m, n, rp, ci, v = CASES["densecol_400x100"]
+    M = swift.SwiftMatrix(m, n, rp, ci, np.asarray(v, np.float32))
+    v2 = np.random.default_rng(9).standard_normal(len(ci)).astype(np.float32)
+    M.update_values(torch.tensor(v2, device="cuda"))
+    B = np.random.default_rng(10).uniform(-1, 1, (n, 32)).astype(np.float32)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, v2.astype(np.float64), B.astype(np.float64))
+    C = M.spmm(torch.tensor(B, device="cuda")).cpu().numpy()
+    assert gpu_common.rel_err(C, C64, absC) <= 1e-5
+    M.close()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_autograd_finite_differences(swift, cuda, fused):
+    """d loss / d vals and d loss / d B of loss = <W, A(vals) B> against FP64
+    central differences on a small power-law graph (induced subgraph size)."""
+    import torch
+    d = swift.generate(4, scale=0.01, seed=1)  # 2,000 nodes
+    m, n, rp, ci, v = d["m"], d["n"], d["rowptr"], d["colidx"], d["val"]
+    K = 16
+    M = swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float64)
+    Fn = swift.spmm_function()
+    rng = np.random.default_rng(0)
+    vals = torch.tensor(v, device="cuda", requires_grad=True)
+    B = torch.tensor(rng.standard_normal((n, K)), device="cuda", requires_grad=True)
+    W = torch.tensor(rng.standard_normal((m, K)), device="cuda")
+    loss = (Fn.apply(vals, B, M, fused) * W).sum()
+    loss.backward()
+    gv, gB = vals.grad.cpu().numpy(), B.grad.cpu().numpy()
+
+    def f(vv, BB):
+        M.update_values(torch.tensor(vv, device="cuda"))
+        return float((M.spmm(torch.tensor(BB, device="cuda")) * W).sum())
+
+    h = 1e-6
+    Bn = B.detach().cpu().numpy()
+    for e in rng.choice(len(v), 6, replace=False):
+        vp, vm = v.copy(), v.copy()
+        vp[e] += h
+        vm[e] -= h
+        fd = (f(vp, Bn) - f(vm, Bn)) / (2 * h)
+        assert abs(fd - gv[e]) <= 1e-6 * max(1.0, abs(fd))
+    for (i, k) in [(0, 0), (n // 2, 3), (n - 1, K - 1)]:
+        Bp, Bm = Bn.copy(), Bn.copy()
+        Bp[i, k] += h
+        Bm[i, k] -= h
+        fd = (f(v, Bp) - f(v, Bm)) / (2 * h)
+        assert abs(fd - gB[i, k]) <= 1e-6 * max(1.0, abs(fd))
+    M.close()

@@ -1,0 +1,223 @@
+"""GPU SpMM parity: every layout / kernel / value type / K against the FP64
+oracle, per element, with the error normalised by (|A||B|)[i,k]:
+    FP32 <= 1e-5, FP64 <= 1e-12, BF16/F16 storage <= 1e-2,
+and against the reference CPU path (Swift_cpu, FP32) to the same bound."""
+import numpy as np
+import pytest
+
+import gpu_common
+
+pytestmark = pytest.mark.gpu
+
+CASES = gpu_common.cases()
+TOL = {"f32": 1e-5, "f64": 1e-12}
+
+
+def run_case(swift, oracle, name, K, dt, layout, kernel, flags=0, seed=0):
+    import torch
+    m, n, rp, ci, v = CASES[name]
+    dtype = torch.float32 if dt == "f32" else torch.float64
+    npdt = np.float32 if dt == "f32" else np.float64
+    vv = np.asarray(v, npdt)
+    rng = np.random.default_rng(seed + K)
+    B = rng.uniform(-1, 1, (n, K)).astype(npdt)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=dtype)
+    Bt = torch.tensor(B, device="cuda")
+    if flags & swift.B_SORTED:
+        Bt = Bt[torch.tensor(M.arrays(["perm"])["perm"].astype(np.int64), device="cuda")]
+    if layout == swift.COL_MAJOR:
+        Bt = Bt.t().contiguous()
+    C = M.spmm(Bt, layout=layout, kernel=kernel, flags=flags)
+    torch.cuda.synchronize()
+    C = C.cpu().numpy()
+    if layout == swift.COL_MAJOR:
+        C = C.T
+    M.close()
+    return C, C64, absC
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("K", [32, 128])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_row_major(swift, oracle, cuda, name, K, dt):
+    print("case", name, K, dt, flush=True)
+    C, C64, absC = run_case(swift, oracle, name, K, dt, swift.ROW_MAJOR, swift.KERNEL_DEFAULT)
+    assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("K", [32, 128])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_col_major(swift, oracle, cuda, name, K, dt):
+    print("case", name, K, dt, flush=True)
+    C, C64, absC = run_case(swift, oracle, name, K, dt, swift.COL_MAJOR, swift.KERNEL_DEFAULT)
+    assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
+
+
+@pytest.mark.parametrize("kernel", ["BALANCED", "UNBALANCED", "KCHUNK"])
+@pytest.mark.parametrize("layout", ["ROW", "COL"])
+@pytest.mark.parametrize("name", ["densecol_400x100", "equalrows_100x70", "powerlaw_3000x2500"])
+def test_kernel_variants(swift, oracle, cuda, kernel, layout, name):
+    print("case", name, kernel, layout, flush=True)
+    if layout == "ROW" and kernel == "KCHUNK":
+        pytest.skip("K chunks are a column-major grid dimension")
+    k = getattr(swift, "KERNEL_" + kernel)
+    lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
+    C, C64, absC = run_case(swift, oracle, name, 64, "f32", lay, k)
+    assert gpu_common.rel_err(C, C64, absC) <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["equalrows_100x70", "powerlaw_3000x2500", "random_300x200"])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_rowsort_kernel(swift, oracle, cuda, name, dt):
+    """Swift_GPU_4's segmented reduction over row-sorted groups of 8 tiles."""
+    import torch
+    print("case", name, dt, flush=True)
+    m, n, rp, ci, v = CASES[name]
+    dtype = torch.float32 if dt == "f32" else torch.float64
+    npdt = np.float32 if dt == "f32" else np.float64
+    vv = np.asarray(v, npdt)
+    B = np.random.default_rng(7).uniform(-1, 1, (n, 96)).astype(npdt)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=dtype)
+    M.row_idx_sort()
+    C = M.spmm(torch.tensor(B.T.copy(), device="cuda"), layout=swift.COL_MAJOR,
+               kernel=swift.KERNEL_ROWSORT).cpu().numpy().T
+    assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
+    M.close()
+
+
+@pytest.mark.parametrize("K", [1, 3, 8, 24, 48, 96, 100, 192, 256, 384, 512, 768])
+@pytest.mark.parametrize("layout", ["ROW", "COL"])
+def test_irregular_k(swift, oracle, cuda, K, layout):
+    """K sweep of test/irregular_K (24..768) plus non-multiples of the vector width."""
+    print("case K", K, layout, flush=True)
+    lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
+    C, C64, absC = run_case(swift, oracle, "powerlaw_500x333", K, "f32", lay,
+                            swift.KERNEL_DEFAULT)
+    assert gpu_common.rel_err(C, C64, absC) <= 1e-5
+
+
+@pytest.mark.parametrize("layout", ["ROW", "COL"])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_b_sorted_flag(swift, oracle, cuda, layout, dt):
+    """B already in sorted column order (the reference's sort_dense_mtx)."""
+    lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
+    C, C64, absC = run_case(swift, oracle, "powerlaw_3000x2500", 32, dt, lay,
+                            swift.KERNEL_DEFAULT, flags=swift.B_SORTED)
+    assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
+
+
+def test_accumulate_flag(swift, oracle, cuda):
+    import torch
+    m, n, rp, ci, v = CASES["powerlaw_500x333"]
+    vv = np.asarray(v, np.float32)
+    B = np.random.default_rng(1).uniform(-1, 1, (n, 32)).astype(np.float32)
+    C0 = np.random.default_rng(2).uniform(-1, 1, (m, 32)).astype(np.float32)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv)
+    out = torch.tensor(C0, device="cuda")
+    M.spmm(torch.tensor(B, device="cuda"), out=out, flags=swift.ACCUMULATE)
+    got = out.cpu().numpy().astype(np.float64)
+    assert gpu_common.rel_err(got - C0, C64, absC + np.abs(C0)) <= 1e-5
+    M.close()
+
+
+@pytest.mark.parametrize("name", ["random_300x200", "densecol_400x100", "powerlaw_3000x2500"])
+def test_matches_reference_cpu_path(swift, oracle, cuda, name):
+    """Against Swift_cpu (src/Swift_cpu.h:31-65) in FP32 on the same inputs."""
+    import torch
+    m, n, rp, ci, v = CASES[name]
+    vv = np.asarray(v, np.float32)
+    f = oracle.build(m, n, rp, ci, vv.astype(np.float64))
+    B = np.random.default_rng(3).uniform(-1, 1, (n, 32)).astype(np.float32)
+    C_cpu = f.swift_cpu(32, B[f["perm"]], np.float32)
+    _, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv)
+    C = M.spmm(torch.tensor(B, device="cuda")).cpu().numpy()
+    assert gpu_common.rel_err(C, C_cpu.astype(np.float64), absC) <= 2e-5
+    M.close()
+
+
+def test_host_launcher(swift, oracle, cuda):
+    """swift_spmm_host keeps the reference launcher contract: sorted host B in,
+    host C out, ResultVerify at 1e-2, ms = -1 on a mismatch."""
+    import ctypes as C
+    m, n, rp, ci, v = CASES["powerlaw_3000x2500"]
+    K = 32
+    vv = np.asarray(v, np.float64)
+    f = oracle.build(m, n, rp, ci, vv)
+    B = np.random.default_rng(4).uniform(-1, 1, (n, K))
+    golden, _ = oracle.spmm_f64(m, n, rp, ci, vv, B)
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=__import__("torch").float64)
+    sB = np.ascontiguousarray(B[f["perm"]])
+    out = np.zeros((m, K))
+    ms, gf = C.c_float(), C.c_double()
+    L = swift.lib()
+    rc = L.swift_spmm_host(M.h, K, swift.ROW_MAJOR, swift.F64, swift.KERNEL_DEFAULT,
+                           sB.ctypes.data, out.ctypes.data, golden.ctypes.data, 1, 3,
+                           C.byref(ms), C.byref(gf))
+    assert rc == 0 and ms.value > 0 and gf.value > 0
+    np.testing.assert_allclose(out, golden, rtol=1e-10, atol=1e-10)
+    bad = golden.copy()
+    bad[0, 0] += 1.0
+    rc = L.swift_spmm_host(M.h, K, swift.ROW_MAJOR, swift.F64, swift.KERNEL_DEFAULT,
+                           sB.ctypes.data, out.ctypes.data, bad.ctypes.data, 0, 1,
+                           C.byref(ms), C.byref(gf))
+    assert rc == -14 and ms.value == -1.0
+    M.close()
+
+
+def test_dtype_mismatch_rejected(swift, cuda):
+    import torch
+    m, n, rp, ci, v = CASES["random_300x200"]
+    M = swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float32)
+    with pytest.raises(swift.SwiftError):
+        M.spmm(torch.zeros((n, 32), dtype=torch.float64, device="cuda"))
+    with pytest.raises(swift.SwiftError):
+        M.spmm(torch.zeros((n, 32), dtype=torch.bfloat16, device="cuda"))  # not prepared
+    M.close()
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("layout", ["ROW", "COL"])
+def test_config2_full(swift, oracle, cuda, layout):
+    """10^6 x 10^6, 16M nonzeros: K=32 against the FP64 oracle element by
+    element; K=128 through the column-sum checksum 1^T C = (1^T A) B and
+    linearity A(B1 + B2) = A B1 + A B2."""
+    import torch
+    d = swift.generate(2, scale=1.0, seed=0)
+    m, n, rp, ci = d["m"], d["n"], d["rowptr"], d["colidx"]
+    vv = d["val"].astype(np.float32)
+    print("config2 full", layout, d["nnz"], flush=True)
+    lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
+    M = swift.SwiftMatrix(m, n, rp, ci, vv)
+    B = swift.generate_dense(n * 32, 0, -1.0, 1.0, seed=5).reshape(n, 32).astype(np.float32)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    Bt = torch.tensor(B if lay == swift.ROW_MAJOR else B.T.copy(), device="cuda")
+    C = M.spmm(Bt, layout=lay).cpu().numpy()
+    if lay == swift.COL_MAJOR:
+        C = C.T
+    assert gpu_common.rel_err(C, C64, absC) <= 1e-5
+    del C64, absC, C
+    K = 128
+    g = torch.Generator(device="cuda").manual_seed(0)
+    B1 = torch.rand((n, K), device="cuda", generator=g) * 2 - 1
+    B2 = torch.rand((n, K), device="cuda", generator=g) * 2 - 1
+    T = (lambda x: x) if lay == swift.ROW_MAJOR else (lambda x: x.t().contiguous())
+    C1 = T(M.spmm(T(B1), layout=lay))
+    C2 = T(M.spmm(T(B2), layout=lay))
+    C12 = T(M.spmm(T(B1 + B2), layout=lay))
+    colsum = np.bincount(ci, weights=vv.astype(np.float64), minlength=n)
+    want = torch.tensor(colsum, device="cuda") @ B1.double()
+    got = C1.double().sum(0)
+    absA = np.bincount(ci, weights=np.abs(vv.astype(np.float64)), minlength=n)
+    scale = torch.tensor(absA, device="cuda") @ B1.double().abs()
+    assert float(((got - want).abs() / (scale + 1e-30)).max()) <= 1e-5
+    Mabs = swift.SwiftMatrix(m, n, rp, ci, np.abs(vv))
+    S = T(Mabs.spmm(T(B1.abs() + B2.abs()), layout=lay)).double()
+    lin = (C12 - C1 - C2).abs().double() / (S + 1e-30)
+    assert float(lin.max()) <= 3e-5
+    Mabs.close()
+    M.close()

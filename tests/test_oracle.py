@@ -1,0 +1,214 @@
+"""The CPU oracle, pinned against the reference's own CPU code.
+
+Golden fixtures (tests/golden/*.npz) hold the outputs of the reference's CPU
+headers compiled in place (oracle/_ref, written by tests/golden/make_golden.py).
+Where those leave something open the checks use what the mathematics fixes:
+the FP64 product against a dense matmul, the tile/reside decomposition
+reproducing the CSR pattern entry for entry, and brute-force sorts.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import matgen
+
+GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+FORMAT_KEYS = ("tilenum", "regular_nnz", "reside_col", "reside_nnz", "segsum", "numblk",
+               "sortrowidx", "tile_ptr", "tile_len", "tile_colidx", "tile_tall",
+               "tile_format", "segmentoffset", "sortrowindex", "reside_cscptr",
+               "reside_cscrowidx", "colorblk", "blkstart", "blkstop", "blkptr")
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(params=GOLD, ids=[os.path.basename(p)[:-4] for p in GOLD])
+def gold(request):
+    return load(request.param)
+
+
+def test_golden_present():
+    assert len(GOLD) >= 6
+
+
+def test_csr2csc_matches_reference(oracle, gold):
+    ptr, row, val, npc = oracle.csr2csc(int(gold["m"]), int(gold["n"]), gold["rowptr"],
+                                        gold["colidx"], gold["val"])
+    assert np.array_equal(ptr, gold["csc_ptr"])
+    assert np.array_equal(row, gold["csc_row"])
+    assert np.array_equal(val, gold["csc_val"])
+    assert np.array_equal(npc, gold["nnzpercol"])
+
+
+def test_col_sort_matches_reference(oracle, gold):
+    n = int(gold["n"])
+    perm, sn, sr, sv = oracle.col_sort(n, gold["nnzpercol"], gold["csc_ptr"], gold["csc_row"],
+                                       gold["csc_val"])
+    # the sorted counts do not depend on the tie rule
+    assert np.array_equal(sn, gold["ref_sortnnz"])
+    # stable tie rule: brute force
+    want = np.array(sorted(range(n), key=lambda j: (gold["nnzpercol"][j], j)), np.int32)
+    assert np.array_equal(perm, want)
+    # the reference's own order differs only inside groups of equal counts
+    ref_perm = gold["ref_colsort_perm"]
+    assert sorted(ref_perm.tolist()) == list(range(n))
+    assert np.array_equal(gold["nnzpercol"][ref_perm], sn)
+    assert np.array_equal(sr, gold["sortrowidx_tmp"])
+
+
+def test_format_matches_reference(oracle, gold):
+    f = oracle.build(int(gold["m"]), int(gold["n"]), gold["rowptr"], gold["colidx"], gold["val"])
+    for k in FORMAT_KEYS:
+        assert np.array_equal(np.asarray(f[k]), np.asarray(gold["fmt_" + k])), k
+    assert np.array_equal(f["sortval"], gold["fmt_sortval"])
+    assert np.array_equal(f["reside_val"], gold["fmt_reside_val"])
+
+
+def test_row_sort_matches_reference(oracle, gold):
+    f = oracle.build(int(gold["m"]), int(gold["n"]), gold["rowptr"], gold["colidx"], gold["val"])
+    f.row_idx_sort()
+    assert np.array_equal(f["rowsorted"], gold["rowsorted"])
+    assert np.array_equal(f["val_offset"], gold["val_offset"])
+    # val_position: a permutation per group that sorts the keys, ties by lane
+    R = f["regular_nnz"]
+    vp = f["val_position"]
+    for g0 in range(0, R, 256):
+        g1 = min(R, g0 + 256)
+        pos = vp[g0:g1]
+        assert sorted(pos.tolist()) == list(range(g1 - g0))
+        keys = f["sortrowidx"][g0:g1]
+        assert np.array_equal(f["rowsorted"][g0:g1][pos], keys)
+        order = np.argsort(pos)
+        assert np.array_equal(order, np.lexsort((np.arange(g1 - g0), keys)))
+
+
+def test_swift_cpu_matches_reference_bitwise(oracle, gold):
+    f = oracle.build(int(gold["m"]), int(gold["n"]), gold["rowptr"], gold["colidx"], gold["val"])
+    K = gold["B"].shape[1]
+    sB = gold["B"][f["perm"]]
+    c64 = f.swift_cpu(K, sB, np.float64)
+    assert np.array_equal(c64, gold["C64"])
+    c32 = f.swift_cpu(K, sB.astype(np.float32), np.float32)
+    assert np.array_equal(c32, gold["C32"])
+
+
+def test_swift_cpu_equals_dense_product(oracle, gold):
+    m, n = int(gold["m"]), int(gold["n"])
+    A = matgen.csr_to_dense(m, n, gold["rowptr"], gold["colidx"], gold["val"])
+    want = A @ gold["B"]
+    np.testing.assert_allclose(gold["C64"], want, rtol=1e-12, atol=1e-12)
+    C, absC = oracle.spmm_f64(m, n, gold["rowptr"], gold["colidx"], gold["val"], gold["B"])
+    np.testing.assert_allclose(C, want, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(absC, np.abs(A) @ np.abs(gold["B"]), rtol=1e-12, atol=1e-12)
+
+
+def decompose(f):
+    """(row, sorted col, value) of every entry the format stores."""
+    n = f["n"]
+    ents = []
+    for t in range(f["tilenum"]):
+        b = f["tile_ptr"][t]
+        for l in range(32):
+            ents.append((f["sortrowidx"][b + l], f["tile_colidx"][t] + l, f["sortval"][b + l]))
+    for r in range(f["reside_col"]):
+        for p in range(f["reside_cscptr"][r], f["reside_cscptr"][r + 1]):
+            ents.append((f["reside_cscrowidx"][p], n - 1 - r, f["reside_val"][p]))
+    return ents
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_format_invariants(oracle, seed):
+    m, n, rp, ci, v = matgen.powerlaw_csr(257, 199, seed)
+    f = oracle.build(m, n, rp, ci, v)
+    nnz = len(ci)
+    assert f["regular_nnz"] + f["reside_nnz"] == nnz
+    assert f["regular_nnz"] == 32 * f["tilenum"]
+    assert 0 <= f["reside_col"] <= 31
+    # the decomposition reproduces the CSR pattern and values exactly
+    got = sorted((int(r), int(f["perm"][c]), float(x)) for r, c, x in decompose(f))
+    want = sorted((i, int(ci[e]), float(v[e])) for i in range(m) for e in range(rp[i], rp[i + 1]))
+    assert got == want
+    # a tile = 32 consecutive sorted columns at one depth
+    sn, cptr = f["sortnnz"], np.concatenate([[0], np.cumsum(f["sortnnz"])])
+    for t in range(f["tilenum"]):
+        c0 = f["tile_colidx"][t]
+        b = f["tile_ptr"][t]
+        depths = set()
+        for l in range(32):
+            pos = np.where((f["sortrowidx_tmp"][cptr[c0 + l]:cptr[c0 + l + 1]] ==
+                            f["sortrowidx"][b + l]))[0]
+            assert len(pos) >= 1
+            depths.add(tuple(pos.tolist()))
+        assert len(depths) >= 1
+    # formats and runs
+    for t in range(f["tilenum"]):
+        rows = f["sortrowidx"][32 * t:32 * t + 32]
+        ch = int(np.sum(rows[1:] != rows[:-1]))
+        assert f["tile_format"][t] == (0 if ch == 0 else (2 if ch < 16 else 1))
+    assert f["segsum"] == f["countrow"][-1]
+    assert int(np.sum(f["segmentoffset"])) == f["regular_nnz"] - f["segsum"]
+    # balance blocks cover each reside column, none longer than 32
+    for b in range(f["numblk"]):
+        u = int(f["colorblk"][b])
+        r = u & 0x7FFFFFFF
+        if u >> 31:
+            assert 0 < f["blkstop"][b] - f["blkstart"][b] <= 32
+        else:
+            assert f["reside_cscptr"][r + 1] - f["reside_cscptr"][r] <= 32
+
+
+def test_no_reside_when_uniform_columns(oracle):
+    # every column has the same count and N % 32 == 0: no irregular part
+    m, n = 64, 64
+    rows = [np.array([(i + d) % n for d in range(3)]) for i in range(m)]
+    rp = np.concatenate([[0], np.cumsum([3] * m)]).astype(np.int32)
+    ci = np.concatenate([np.sort(r) for r in rows]).astype(np.int32)
+    f = oracle.build(m, n, rp, ci, np.ones(len(ci)))
+    assert f["reside_nnz"] == 0 and f["reside_col"] == 0
+    assert f["tilenum"] == 3 * 64 // 32
+
+
+def test_narrow_matrix_all_reside(oracle):
+    # N < 32: no tile fits, everything is irregular
+    m, n, rp, ci, v = matgen.random_csr(50, 20, 0.3, 3)
+    f = oracle.build(m, n, rp, ci, v)
+    assert f["tilenum"] == 0
+    assert f["reside_nnz"] == len(ci)
+    assert f["reside_col"] == 20 or f["reside_col"] == int(np.count_nonzero(np.bincount(ci, minlength=n) > 0))
+
+
+def test_empty_matrix(oracle):
+    f = oracle.build(10, 40, np.zeros(11, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    assert f["tilenum"] == 0 and f["reside_nnz"] == 0 and f["numblk"] == 0
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_oracle_against_live_reference(oracle, seed):
+    if not oracle.Ref.available(np.float64):
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    ref = oracle.Ref(np.float64)
+    m, n, rp, ci, v = matgen.dense_column_csr(300 + seed, 90 + 7 * seed, 100 + seed)
+    f = oracle.build(m, n, rp, ci, v)
+    h, r = ref.format(m, n, f["sortrowidx_tmp"], f["sortval_tmp"], f["sortnnz"])
+    for k in FORMAT_KEYS:
+        assert np.array_equal(np.asarray(r[k]), np.asarray(f[k])), k
+    ref.free(h)
+
+
+def test_planted_mistake_is_caught(oracle):
+    """Flip one tile entry of the oracle's output: the decomposition check and
+    the Swift_cpu comparison both see it."""
+    g = load(GOLD[0])
+    f = oracle.build(int(g["m"]), int(g["n"]), g["rowptr"], g["colidx"], g["val"])
+    bad = dict(f.a)
+    bad["sortrowidx"] = bad["sortrowidx"].copy()
+    bad["sortrowidx"][5] = (bad["sortrowidx"][5] + 1) % int(g["m"])
+    assert not np.array_equal(bad["sortrowidx"], g["fmt_sortrowidx"])
+    got = sorted((int(r), int(f["perm"][c])) for r, c, _ in decompose(bad))
+    want = sorted((i, int(g["colidx"][e])) for i in range(int(g["m"]))
+                  for e in range(g["rowptr"][i], g["rowptr"][i + 1]))
+    assert got != want
