@@ -155,7 +155,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                     "kernel": "k_regular_row (regular tiles)",
+                     "kernel": "k_rowview_row (regular part, row view)",
                      "algorithmic_bytes": alg},
         "gpu": torch.cuda.get_device_name(0),
     }
@@ -193,6 +193,8 @@ def extras(S, torch, M, d, B, K):
     C = torch.empty((m, K), device="cuda", dtype=torch.float32)
     base = time_call(torch, lambda: M.spmm(B, out=C))
     res["f32_row_ms"] = round(base, 4)
+    res["f32_row_tiles_scatter_ms"] = round(time_call(
+        torch, lambda: M.spmm(B, out=C, kernel=S.KERNEL_BALANCED), reps=3, warm=1), 4)
     perm = torch.tensor(M.arrays(["perm"])["perm"].astype(np.int64), device="cuda")
     Bs = B[perm].contiguous()
     res["f32_row_Bsorted_ms"] = round(time_call(torch, lambda: M.spmm(Bs, out=C,

@@ -49,8 +49,11 @@ typedef enum {
     SWIFT_KERNEL_BALANCED = 1,   /* reside part per balance block (Swift_GPU / Swift_gpu2) */
     SWIFT_KERNEL_UNBALANCED = 2, /* reside part per column (Swift_gpu1)                    */
     SWIFT_KERNEL_KCHUNK = 3,     /* 2-D grid tiles x K-chunks (Swift_GPU_3)                */
-    SWIFT_KERNEL_ROWSORT = 4     /* same-row segmented reduction over row-sorted groups
+    SWIFT_KERNEL_ROWSORT = 4,    /* same-row segmented reduction over row-sorted groups
                                     of 8 tiles (Swift_GPU_4); needs swift_row_idx_sort */
+    SWIFT_KERNEL_ROWVIEW = 5     /* row-major: regular part walked output-stationary
+                                    through its row view (merge-path balanced), reside
+                                    part in balance blocks; the row-major default    */
 } swift_kernel;
 
 /* swift_spmm flags */

@@ -73,6 +73,9 @@ static int update_values_t(swift_matrix *M, const T *csr, hipStream_t s) {
     if (M->nnz)
         hipLaunchKernelGGL(k_gather_vals<T>, dim3(g1(M->nnz)), dim3(256), 0, s, M->sortsrc_tmp, csr,
                            (size_t)M->nnz, (T *)M->sortval_tmp);
+    if (M->regular_nnz)
+        hipLaunchKernelGGL(k_gather_vals<T>, dim3(g1(M->regular_nnz)), dim3(256), 0, s, M->rv_src,
+                           csr, (size_t)M->regular_nnz, (T *)M->rv_val);
     SWIFT_CHECK_LAUNCH();
     return SWIFT_OK;
 }

@@ -449,6 +449,67 @@ __global__ void k_work_fill(int nwork, int nsteps, const int *__restrict__ woff,
     work[w] = make_int2(start, cnt);
 }
 
+// ------------------------------------------------------------ row view --
+
+__global__ void k_row_hist(const int *__restrict__ rows, int n, int *__restrict__ cnt) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) atomicAdd(&cnt[rows[p]], 1);
+}
+
+// q-th entry of the row view <- regular entry p = order[q]: its B row is the
+// original column of sorted column tile_colidx[p/32] + p%32.
+__global__ void k_rv_fill(int R, const unsigned *__restrict__ order,
+                          const int *__restrict__ tile_colidx, const int *__restrict__ perm,
+                          const void *__restrict__ sval, const int *__restrict__ ssrc, int vs,
+                          int *__restrict__ rv_col, void *__restrict__ rv_val,
+                          int *__restrict__ rv_src) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= R) return;
+    int p = (int)order[q];
+    int c = tile_colidx[p >> 5] + (p & 31);
+    rv_col[q] = perm[c];
+    copy_value(vs, rv_val, q, sval, p);
+    rv_src[q] = ssrc[p];
+}
+
+__global__ void k_inv_perm(const int *__restrict__ perm, int n, int *__restrict__ inv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[perm[i]] = i;
+}
+
+// Merge path over rows and entries: the end of row i is item rv_ptr[i+1] + i,
+// entry e of row r is item e + r.  Segment s starts at item d = s*W; the rows
+// finished before d are those with rv_ptr[i+1] + i < d.
+__global__ void k_mp_seg(int m, int R, int W, int nseg, const int *__restrict__ ptr,
+                         int2 *__restrict__ seg) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nseg) return;
+    long long d = min((long long)s * W, (long long)m + R);
+    int lo = 0, hi = m;  // count of i in [0, m) with ptr[i+1] + i < d
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if ((long long)ptr[mid + 1] + mid < d) lo = mid + 1; else hi = mid;
+    }
+    seg[s] = make_int2(lo, (int)(d - lo));
+}
+
+// A segment ending inside row r (entries of r before and after its end) makes
+// r a split row: its partial sums are added atomically onto a zeroed row.
+__global__ void k_mp_split_flag(int m, int nseg, const int *__restrict__ ptr,
+                                const int2 *__restrict__ seg, int *__restrict__ flag) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    int2 end = seg[s + 1];
+    flag[s] = (end.x < m && end.y > ptr[end.x]) ? 1 : 0;
+}
+
+__global__ void k_mp_split_compact(int nseg, const int2 *__restrict__ seg,
+                                   const int *__restrict__ flag, const int *__restrict__ idx,
+                                   int *__restrict__ rows) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nseg && flag[s]) rows[idx[s]] = seg[s + 1].x;
+}
+
 // ------------------------------------------------------------ row sort --
 
 // Groups of 8 tiles = 256 entries (the last group holds tilenum%8 tiles).
@@ -521,7 +582,8 @@ static void free_matrix_arrays(swift_matrix *M) {
                     M->reside_cscrowidx, M->reside_val, M->reside_src, M->colorblk,
                     M->blkstart, M->blkstop, M->blkptr, M->val_position, M->val_offset,
                     M->rowsorted, M->rowsort_src, M->csr_rowptr, M->csr_colidx,
-                    M->sortval_lp, M->reside_val_lp, M->work};
+                    M->sortval_lp, M->reside_val_lp, M->work, M->rv_ptr, M->rv_col,
+                    M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->seg, M->split_rows};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     M->scratch.release();
@@ -551,6 +613,8 @@ static int dalloc_bytes(void **p, size_t bytes) {
     }
     return SWIFT_OK;
 }
+
+int build_rowview(swift_matrix *M, hipStream_t s);
 
 int build_work_list(swift_matrix *M, int depth, hipStream_t s) {
     if (M->work) { (void)hipFree(M->work); M->work = nullptr; }
@@ -626,7 +690,7 @@ int build_impl(swift_matrix *M, const int *d_rowptr, const int *d_colidx,
                                 M->perm, M->cptr, &M->max_col_nnz, T, s));
         SWIFT_HIP(hipStreamSynchronize(s));
     }
-    if (n == 0) return SWIFT_OK;
+    if (n == 0) return build_rowview(M, s);
     // 3. steps
     Buffer flag, idx, ntile, sw, tmp, small;
     SWIFT_TRY(flag.alloc(sizeof(int) * ((size_t)n + 1)));
@@ -748,10 +812,89 @@ int build_impl(swift_matrix *M, const int *d_rowptr, const int *d_colidx,
     SWIFT_HIP(hipMemcpyAsync(&numblk, small.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     SWIFT_HIP(hipStreamSynchronize(s));
     M->numblk = numblk;
-    // 8. kernel work list (column tiles)
+    // 8. kernel schedules: column-tile work list and the row view
     SWIFT_TRY(build_work_list(M, 16, s));
+    SWIFT_TRY(build_rowview(M, s));
     SWIFT_HIP(hipStreamSynchronize(s));
     return SWIFT_OK;
+}
+
+int build_segments(swift_matrix *M, int W, hipStream_t s) {
+    if (M->seg) { (void)hipFree(M->seg); M->seg = nullptr; }
+    if (M->split_rows) { (void)hipFree(M->split_rows); M->split_rows = nullptr; }
+    const long long items = (long long)M->m + M->regular_nnz;
+    const int nseg = (int)((items + W - 1) / W);
+    M->mp_items = W;
+    M->nseg = nseg;
+    M->nsplit = 0;
+    SWIFT_TRY(dalloc(&M->seg, (size_t)nseg + 1));
+    hipLaunchKernelGGL(k_mp_seg, dim3(grid1((size_t)nseg + 1)), dim3(256), 0, s, M->m,
+                       M->regular_nnz, W, nseg, M->rv_ptr, M->seg);
+    SWIFT_CHECK_LAUNCH();
+    if (nseg == 0) return SWIFT_OK;
+    Buffer flag, idx, tmp;
+    SWIFT_TRY(flag.alloc(sizeof(int) * ((size_t)nseg + 1)));
+    SWIFT_TRY(idx.alloc(sizeof(int) * ((size_t)nseg + 1)));
+    SWIFT_TRY(tmp.alloc(scan_temp_bytes((size_t)nseg)));
+    hipLaunchKernelGGL(k_mp_split_flag, dim3(grid1(nseg)), dim3(256), 0, s, M->m, nseg, M->rv_ptr,
+                       M->seg, flag.as<int>());
+    SWIFT_CHECK_LAUNCH();
+    SWIFT_TRY(exclusive_scan(flag.as<int>(), idx.as<int>(), (size_t)nseg, idx.as<int>() + nseg,
+                             tmp.ptr, s));
+    int nsplit = 0;
+    SWIFT_HIP(hipMemcpyAsync(&nsplit, idx.as<int>() + nseg, sizeof(int), hipMemcpyDeviceToHost, s));
+    SWIFT_HIP(hipStreamSynchronize(s));
+    SWIFT_TRY(dalloc(&M->split_rows, (size_t)nsplit));
+    if (nsplit) {
+        hipLaunchKernelGGL(k_mp_split_compact, dim3(grid1(nseg)), dim3(256), 0, s, nseg, M->seg,
+                           flag.as<int>(), idx.as<int>(), M->split_rows);
+        SWIFT_CHECK_LAUNCH();
+    }
+    SWIFT_HIP(hipStreamSynchronize(s));
+    M->nsplit = nsplit;
+    return SWIFT_OK;
+}
+
+// Row view of the regular part: stable radix sort of the regular entries by
+// row, so each row keeps its entries in tile order.
+int build_rowview(swift_matrix *M, hipStream_t s) {
+    const int m = M->m, R = M->regular_nnz;
+    const int vs = vsize_of(M->vtype);
+    SWIFT_TRY(dalloc(&M->rv_ptr, (size_t)m + 1));
+    SWIFT_TRY(dalloc(&M->rv_col, (size_t)R));
+    SWIFT_TRY(dalloc(&M->rv_src, (size_t)R));
+    SWIFT_TRY(dalloc_bytes(&M->rv_val, (size_t)R * vs));
+    SWIFT_TRY(dalloc(&M->inv_perm, (size_t)M->n));
+    if (M->n) {
+        hipLaunchKernelGGL(k_inv_perm, dim3(grid1(M->n)), dim3(256), 0, s, M->perm, M->n,
+                           M->inv_perm);
+        SWIFT_CHECK_LAUNCH();
+    }
+    Buffer cnt, tmp, keys, vals, kalt, valt;
+    SWIFT_TRY(cnt.alloc(sizeof(int) * ((size_t)m + 1)));
+    SWIFT_TRY(fill_i32(cnt.as<int>(), 0, (size_t)m + 1, s));
+    if (R) {
+        hipLaunchKernelGGL(k_row_hist, dim3(grid1(R)), dim3(256), 0, s, M->sortrowidx, R,
+                           cnt.as<int>());
+        SWIFT_CHECK_LAUNCH();
+    }
+    SWIFT_TRY(tmp.alloc(std::max(scan_temp_bytes((size_t)m + 1), radix_temp_bytes((size_t)R))));
+    SWIFT_TRY(exclusive_scan(cnt.as<int>(), M->rv_ptr, (size_t)m, M->rv_ptr + m, tmp.ptr, s));
+    if (R) {
+        size_t b = sizeof(unsigned) * (size_t)R;
+        SWIFT_TRY(keys.alloc(b)); SWIFT_TRY(vals.alloc(b));
+        SWIFT_TRY(kalt.alloc(b)); SWIFT_TRY(valt.alloc(b));
+        SWIFT_TRY(radix_sort_pairs((const unsigned *)M->sortrowidx, nullptr, keys.as<unsigned>(),
+                                   vals.as<unsigned>(), kalt.as<unsigned>(), valt.as<unsigned>(),
+                                   (size_t)R, bits_for((unsigned)(m > 0 ? m - 1 : 0)), tmp.ptr,
+                                   s));
+        hipLaunchKernelGGL(k_rv_fill, dim3(grid1(R)), dim3(256), 0, s, R, vals.as<unsigned>(),
+                           M->tile_colidx, M->perm, M->sortval, M->sortsrc, vs, M->rv_col,
+                           M->rv_val, M->rv_src);
+        SWIFT_CHECK_LAUNCH();
+    }
+    SWIFT_HIP(hipStreamSynchronize(s));
+    return build_segments(M, 128, s);
 }
 
 int row_sort_impl(swift_matrix *M, hipStream_t s) {

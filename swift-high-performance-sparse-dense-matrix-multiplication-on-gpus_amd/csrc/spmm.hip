@@ -78,21 +78,37 @@ __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(
 __device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
 
 // ---------------------------------------------------- row-major kernels --
+//
+// Lanes run along K in groups of LPE lanes (one entry stream per group, EPS =
+// 64/LPE groups per wave), VEC elements per lane per chunk, Q chunks.  Two
+// element layouts are used:
+//   strided     k = q*LPE*VEC + v*LPE + sub   (atomics: for a fixed (q, v) the
+//               LPE lanes hit LPE consecutive elements, i.e. whole lines)
+//   contiguous  k = q*LPE*VEC + sub*VEC + v   (16-byte loads and stores)
 
 struct RowArgs {
+    // tile-order kernels
     int nwork;
-    const int2 *work;           // regular: (first tile, tile count)
+    const int2 *work;           // (first tile, tile count) per work item
     const int *tile_colidx;
     const int *srow;            // sortrowidx
     const void *sval;           // sortval (TV)
-    // reside
+    // reside (irregular) kernels
     int nblk;                   // balance blocks (or reside columns)
     const unsigned *colorblk;   // nullptr => unbalanced, one block per column
     const int *blkstart, *blkstop;
     const int *rptr, *rrow;
     const void *rval;
+    // row-view kernel
+    int m, nseg;
+    const int2 *seg;
+    const int *rv_ptr, *rv_col;
+    const void *rv_val;
+    const int *inv_perm;        // B in sorted order: B row of column c is inv_perm[c]
+    int accumulate;
+    // shared
     int n;                      // columns of A
-    const int *perm;            // nullptr when B is already in sorted order
+    const int *perm;            // tile-order: nullptr when B is already sorted
     const void *B;
     void *C;
     int ldb, ldc;               // row strides (elements)
@@ -101,48 +117,43 @@ struct RowArgs {
     float gscale;               // F16: A-value scale
 };
 
+template <class TC>
+__device__ __forceinline__ TC out_scale(const RowArgs &a, int k) {
+    return a.kscale ? (TC)(a.kscale[a.kbase + k] * a.gscale) : (TC)a.gscale;
+}
+
 template <class TC, int VEC, int Q, int LPE, bool MASK>
-__device__ __forceinline__ void flush_row(const RowArgs &a, int row, int sub,
-                                          TC (&acc)[Q][VEC]) {
+__device__ __forceinline__ void flush_strided(const RowArgs &a, int row, int sub,
+                                              TC (&acc)[Q][VEC]) {
     TC *crow = static_cast<TC *>(a.C) + (size_t)row * a.ldc + a.kbase;
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
+    for (int q = 0; q < Q; q++)
 #pragma unroll
         for (int v = 0; v < VEC; v++) {
-            int k = q * LPE * VEC + sub * VEC + v;
-            if (!MASK || k < a.kcount) {
-                TC x = acc[q][v];
-                if (a.kscale) x *= (TC)(a.kscale[a.kbase + k] * a.gscale);
-                else if (a.gscale != 1.0f) x *= (TC)a.gscale;
-                atomic_add(crow + k, x);
-            }
+            const int k = q * LPE * VEC + v * LPE + sub;
+            if (!MASK || k < a.kcount) atomic_add(crow + k, acc[q][v] * out_scale<TC>(a, k));
             acc[q][v] = (TC)0;
         }
-    }
 }
 
 template <class TB, class TC, int VEC, int Q, int LPE, bool MASK>
-__device__ __forceinline__ void fma_row(const RowArgs &a, int bcol, int sub, TC val,
-                                        TC (&acc)[Q][VEC]) {
+__device__ __forceinline__ void load_strided(const RowArgs &a, int bcol, int sub,
+                                             TC (&b)[Q][VEC]) {
     const TB *brow = static_cast<const TB *>(a.B) + (size_t)bcol * a.ldb + a.kbase;
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-        int k0 = q * LPE * VEC + sub * VEC;
-        TC b[VEC];
-        if (!MASK || k0 < a.kcount) {
-            load_b<TB, VEC, TC>(brow + k0, b);
-        } else {
+    for (int q = 0; q < Q; q++)
 #pragma unroll
-            for (int v = 0; v < VEC; v++) b[v] = (TC)0;
+        for (int v = 0; v < VEC; v++) {
+            const int k = q * LPE * VEC + v * LPE + sub;
+            b[q][v] = (!MASK || k < a.kcount) ? to_acc(brow[k]) : (TC)0;
         }
-#pragma unroll
-        for (int v = 0; v < VEC; v++) acc[q][v] += val * b[v];
-    }
 }
 
-// One wave per work item of up to `depth` tiles of one column block.
+// Tile order, one wave per work item (up to 16 depth tiles of one column
+// block, whose B rows stay hot in L1/L2).  Register accumulators per group
+// and C atomics only when the row changes.
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
-__global__ __launch_bounds__(256) void k_regular_row(RowArgs a) {
+__global__ __launch_bounds__(256) void k_tiles_row(RowArgs a) {
     constexpr int EPS = 64 / LPE;       // entries in flight per wave
     constexpr int STEPS = TILE / EPS;   // steps per tile
     const int lane = threadIdx.x & 63;
@@ -171,17 +182,22 @@ __global__ __launch_bounds__(256) void k_regular_row(RowArgs a) {
             const TC v = __shfl(myval, l, 64);
             const int bc = __shfl(mycol, l, 64);
             if (r != cur) {
-                if (cur >= 0) flush_row<TC, VEC, Q, LPE, MASK>(a, cur, sub, acc);
+                if (cur >= 0) flush_strided<TC, VEC, Q, LPE, MASK>(a, cur, sub, acc);
                 cur = r;
             }
-            fma_row<TB, TC, VEC, Q, LPE, MASK>(a, bc, sub, v, acc);
+            TC b[Q][VEC];
+            load_strided<TB, TC, VEC, Q, LPE, MASK>(a, bc, sub, b);
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int x = 0; x < VEC; x++) acc[q][x] += v * b[q][x];
         }
     }
-    if (cur >= 0) flush_row<TC, VEC, Q, LPE, MASK>(a, cur, sub, acc);
+    if (cur >= 0) flush_strided<TC, VEC, Q, LPE, MASK>(a, cur, sub, acc);
 }
 
 // One wave per balance block (or per reside column when unbalanced): the
-// block's B row is loaded once, each entry is one row flush.
+// block's single B row is loaded once, each entry is one row flush.
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
 __global__ __launch_bounds__(256) void k_reside_row(RowArgs a) {
     constexpr int EPS = 64 / LPE;
@@ -203,16 +219,8 @@ __global__ __launch_bounds__(256) void k_reside_row(RowArgs a) {
     }
     const int jj = a.n - 1 - r;
     const int bcol = a.perm ? a.perm[jj] : jj;
-    const TB *brow = static_cast<const TB *>(a.B) + (size_t)bcol * a.ldb + a.kbase;
     TC b[Q][VEC];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        int k0 = q * LPE * VEC + sub * VEC;
-        if (!MASK || k0 < a.kcount) load_b<TB, VEC, TC>(brow + k0, b[q]);
-        else
-#pragma unroll
-            for (int v = 0; v < VEC; v++) b[q][v] = (TC)0;
-    }
+    load_strided<TB, TC, VEC, Q, LPE, MASK>(a, bcol, sub, b);
     const TV *rval = static_cast<const TV *>(a.rval);
     for (int e = start + h; e < stop; e += EPS) {
         const int row = a.rrow[e];
@@ -222,8 +230,141 @@ __global__ __launch_bounds__(256) void k_reside_row(RowArgs a) {
         for (int q = 0; q < Q; q++)
 #pragma unroll
             for (int v = 0; v < VEC; v++) acc[q][v] = val * b[q][v];
-        flush_row<TC, VEC, Q, LPE, MASK>(a, row, sub, acc);
+        flush_strided<TC, VEC, Q, LPE, MASK>(a, row, sub, acc);
     }
+}
+
+// -------------------------------------------------------- row view kernel --
+
+template <class TC, int VEC>
+__device__ __forceinline__ void store_vec(TC *p, const TC (&x)[VEC]) {
+    if constexpr (std::is_same<TC, float>::value && VEC % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < VEC; i += 4)
+            *reinterpret_cast<float4 *>(p + i) = make_float4(x[i], x[i + 1], x[i + 2], x[i + 3]);
+    } else if constexpr (std::is_same<TC, double>::value && VEC % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < VEC; i += 2)
+            *reinterpret_cast<double2 *>(p + i) = make_double2(x[i], x[i + 1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < VEC; i++) p[i] = x[i];
+    }
+}
+
+// Writes one finished row: plain (or read-modify-write when accumulating)
+// vector stores, or atomics for a row split across segments.
+template <class TC, int VEC, int Q, int LPE, bool MASK>
+__device__ __forceinline__ void put_row(const RowArgs &a, int row, int sub, bool split,
+                                        TC (&acc)[Q][VEC]) {
+    TC *crow = static_cast<TC *>(a.C) + (size_t)row * a.ldc + a.kbase;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int k0 = q * LPE * VEC + sub * VEC;
+        TC x[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; v++) x[v] = acc[q][v] * out_scale<TC>(a, k0 + v);
+        if (MASK) {
+#pragma unroll
+            for (int v = 0; v < VEC; v++)
+                if (k0 + v < a.kcount) {
+                    if (split) atomic_add(crow + k0 + v, x[v]);
+                    else crow[k0 + v] = a.accumulate ? crow[k0 + v] + x[v] : x[v];
+                }
+        } else if (split) {
+#pragma unroll
+            for (int v = 0; v < VEC; v++) atomic_add(crow + k0 + v, x[v]);
+        } else {
+            if (a.accumulate) {
+                TC old[VEC];
+                load_b<TC, VEC, TC>(crow + k0, old);
+#pragma unroll
+                for (int v = 0; v < VEC; v++) x[v] += old[v];
+            }
+            store_vec<TC, VEC>(crow + k0, x);
+        }
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+    }
+}
+
+// Merge-path walk of the row view: each group of LPE lanes owns one segment of
+// W items (rows + entries), so every group does the same amount of work
+// whatever the row lengths.  Rows finished inside the segment are stored;
+// rows cut by a segment boundary are split rows (zeroed beforehand) and take
+// atomic adds.  Entries are consumed four at a time so that four 16-byte B
+// row loads are in flight per lane.
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+__global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
+    constexpr int G = 64 / LPE;
+    const int lane = threadIdx.x & 63;
+    const int grp = lane / LPE, sub = lane % LPE;
+    const int sid = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6) * G + grp;
+    if (sid >= a.nseg) return;
+    const int2 st = a.seg[sid], en = a.seg[sid + 1];
+    const int r1 = en.x, e1 = en.y;
+    int row = st.x, e = st.y;
+    const bool carried_in = row < a.m && e > a.rv_ptr[row];
+    const TV *val = static_cast<const TV *>(a.rv_val);
+    const TB *B = static_cast<const TB *>(a.B);
+    TC acc[Q][VEC];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+    while (true) {
+        const int row_end = row < r1 ? a.rv_ptr[row + 1] : e1;
+        while (e < row_end) {
+            const int nb = min(LPE, row_end - e);
+            int mc = 0;
+            TC mv = (TC)0;
+            if (sub < nb) {
+                const int c = a.rv_col[e + sub];
+                mc = a.inv_perm ? a.inv_perm[c] : c;
+                mv = to_acc(val[e + sub]);
+            }
+            for (int j = 0; j < nb; j += 4) {
+                TC b[4][Q][VEC];
+                TC vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int c = __shfl(mc, j + u, LPE);
+                    vv[u] = __shfl(mv, j + u, LPE);
+                    const TB *brow = B + (size_t)c * a.ldb + a.kbase;
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        const int k0 = q * LPE * VEC + sub * VEC;
+                        if (j + u < nb && (!MASK || k0 < a.kcount))
+                            load_b<TB, VEC, TC>(brow + k0, b[u][q]);
+                        else
+#pragma unroll
+                            for (int v = 0; v < VEC; v++) b[u][q][v] = (TC)0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int q = 0; q < Q; q++)
+#pragma unroll
+                        for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * b[u][q][v];
+            }
+            e += nb;
+        }
+        if (row >= r1) break;
+        put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, row == st.x && carried_in, acc);
+        row++;
+    }
+    if (r1 < a.m && e1 > a.rv_ptr[r1]) put_row<TC, VEC, Q, LPE, MASK>(a, r1, sub, true, acc);
+}
+
+// Zeroes C[row, kbase : kbase+kcount) of each split row; one wave per row.
+template <class TC>
+__global__ __launch_bounds__(256) void k_zero_rows(const int *__restrict__ rows, int nrows,
+                                                   TC *__restrict__ C, int ldc, int K) {
+    const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (i >= nrows) return;
+    TC *c = C + (size_t)rows[i] * ldc;
+    for (int k = threadIdx.x & 63; k < K; k += 64) c[k] = (TC)0;
 }
 
 // ---------------------------------------------------- col-major kernels --
@@ -430,17 +571,40 @@ static std::vector<std::pair<int, RowTarget>> plan_row_slices(int K, int vec, bo
     return out;
 }
 
-template <class TV, class TB, class TC, int VEC>
-static int launch_row_vec(const RowArgs &a0, const RowTarget &t, bool regular, hipStream_t s) {
-    const unsigned nitems = regular ? (unsigned)a0.nwork : (unsigned)a0.nblk;
-    if (nitems == 0) return SWIFT_OK;
-    dim3 grid(ceil_div((size_t)nitems * 64, 256)), block(256);
-#define SWIFT_ROW_CASE(Q_, L_)                                                        \
-    if (t.q == Q_ && t.lpe == L_) {                                                   \
-        if (regular) hipLaunchKernelGGL((k_regular_row<TV, TB, TC, VEC, Q_, L_, false>), grid, block, 0, s, a0); \
-        else hipLaunchKernelGGL((k_reside_row<TV, TB, TC, VEC, Q_, L_, false>), grid, block, 0, s, a0); \
-        SWIFT_CHECK_LAUNCH();                                                         \
-        return SWIFT_OK;                                                              \
+enum RowKind { ROW_TILES = 0, ROW_RESIDE = 1, ROW_VIEW = 2 };
+
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+static void launch_row_shape(const RowArgs &a, RowKind kind, hipStream_t s) {
+    if (kind == ROW_VIEW) {
+        constexpr int G = 64 / LPE;
+        const size_t waves = ((size_t)a.nseg + G - 1) / G;
+        if (waves)
+            hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK>),
+                               dim3(ceil_div(waves * 64, 256)), dim3(256), 0, s, a);
+        return;
+    }
+    const unsigned items = kind == ROW_TILES ? (unsigned)a.nwork : (unsigned)a.nblk;
+    if (!items) return;
+    dim3 grid(ceil_div((size_t)items * 64, 256)), block(256);
+    if (kind == ROW_TILES)
+        hipLaunchKernelGGL((k_tiles_row<TV, TB, TC, VEC, Q, LPE, MASK>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_reside_row<TV, TB, TC, VEC, Q, LPE, MASK>), grid, block, 0, s, a);
+}
+
+template <class TV, class TB, class TC>
+static int launch_row(const RowArgs &a, const RowTarget &t, RowKind kind, hipStream_t s) {
+    constexpr int V = Vec16<TB>::N;
+    if (t.mask) {
+        launch_row_shape<TV, TB, TC, 1, 1, 64, true>(a, kind, s);
+        SWIFT_CHECK_LAUNCH();
+        return SWIFT_OK;
+    }
+#define SWIFT_ROW_CASE(Q_, L_)                                                \
+    if (t.q == Q_ && t.lpe == L_) {                                           \
+        launch_row_shape<TV, TB, TC, V, Q_, L_, false>(a, kind, s);           \
+        SWIFT_CHECK_LAUNCH();                                                 \
+        return SWIFT_OK;                                                      \
     }
     SWIFT_ROW_CASE(4, 64)
     SWIFT_ROW_CASE(2, 64)
@@ -448,31 +612,19 @@ static int launch_row_vec(const RowArgs &a0, const RowTarget &t, bool regular, h
     SWIFT_ROW_CASE(1, 32)
     SWIFT_ROW_CASE(1, 16)
     SWIFT_ROW_CASE(1, 8)
-    if constexpr (VEC == 8) { SWIFT_ROW_CASE(1, 4) }
+    if constexpr (V == 8) { SWIFT_ROW_CASE(1, 4) }
 #undef SWIFT_ROW_CASE
     set_error("no row kernel for vec=%d q=%d lpe=%d", t.vec, t.q, t.lpe);
     return SWIFT_ERR_UNSUPPORTED;
 }
 
-template <class TV, class TB, class TC>
-static int launch_row(const RowArgs &a0, const RowTarget &t, bool regular, hipStream_t s) {
-    constexpr int V = Vec16<TB>::N;
-    if (t.mask) {
-        const unsigned nitems = regular ? (unsigned)a0.nwork : (unsigned)a0.nblk;
-        if (nitems == 0) return SWIFT_OK;
-        dim3 grid(ceil_div((size_t)nitems * 64, 256)), block(256);
-        if (regular) hipLaunchKernelGGL((k_regular_row<TV, TB, TC, 1, 1, 64, true>), grid, block, 0, s, a0);
-        else hipLaunchKernelGGL((k_reside_row<TV, TB, TC, 1, 1, 64, true>), grid, block, 0, s, a0);
-        SWIFT_CHECK_LAUNCH();
-        return SWIFT_OK;
-    }
-    return launch_row_vec<TV, TB, TC, V>(a0, t, regular, s);
-}
-
+// Row-major C = A*B.  DEFAULT / ROWVIEW: row view (regular part, output
+// stationary) then the balanced reside blocks.  BALANCED / UNBALANCED: the
+// reference's tile order (zero C, tile kernel, reside kernel).
 template <class TV, class TB, class TC>
 static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned flags,
                     const void *B, void *C, const void *sval, const void *rval,
-                    const float *kscale, float gscale, hipStream_t s) {
+                    const void *rv_val, const float *kscale, float gscale, hipStream_t s) {
     RowArgs a{};
     a.nwork = M->nwork;
     a.work = M->work;
@@ -487,8 +639,17 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.rptr = M->reside_cscptr;
     a.rrow = M->reside_cscrowidx;
     a.rval = rval;
+    a.m = M->m;
+    a.nseg = M->nseg;
+    a.seg = M->seg;
+    a.rv_ptr = M->rv_ptr;
+    a.rv_col = M->rv_col;
+    a.rv_val = rv_val;
+    const bool sorted = (flags & SWIFT_B_SORTED) != 0;
+    a.inv_perm = sorted ? M->inv_perm : nullptr;
+    a.accumulate = (flags & SWIFT_ACCUMULATE) ? 1 : 0;
     a.n = M->n;
-    a.perm = (flags & SWIFT_B_SORTED) ? nullptr : M->perm;
+    a.perm = sorted ? nullptr : M->perm;
     a.B = B;
     a.C = C;
     a.ldb = K;
@@ -496,20 +657,31 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.kscale = kscale;
     a.gscale = gscale;
     constexpr int V = Vec16<TB>::N;
-    const bool aligned = (K % V) == 0 && ((uintptr_t)B % 16) == 0;
+    const bool aligned = (K % V) == 0 && ((uintptr_t)B % 16) == 0 && ((uintptr_t)C % 16) == 0;
     const auto slices = plan_row_slices(K, V, aligned);
+    const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW;
+    if (view) {
+        if (!a.accumulate && M->nsplit) {
+            hipLaunchKernelGGL(k_zero_rows<TC>, dim3(ceil_div((size_t)M->nsplit * 64, 256)),
+                               dim3(256), 0, s, M->split_rows, M->nsplit, (TC *)C, K, K);
+            SWIFT_CHECK_LAUNCH();
+        }
+    } else if (!a.accumulate) {
+        SWIFT_TRY(fill_bytes_zero(C, (size_t)M->m * K * sizeof(TC), s));
+    }
+    mark(1, s);
     for (auto &sl : slices) {
         a.kbase = sl.first;
         const RowTarget &t = sl.second;
         a.kcount = t.mask ? std::min(64, K - sl.first) : t.vec * t.lpe * t.q;
-        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, true, s)));
+        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, view ? ROW_VIEW : ROW_TILES, s)));
     }
     mark(2, s);
     for (auto &sl : slices) {
         a.kbase = sl.first;
         const RowTarget &t = sl.second;
         a.kcount = t.mask ? std::min(64, K - sl.first) : t.vec * t.lpe * t.q;
-        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, false, s)));
+        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, ROW_RESIDE, s)));
     }
     mark(3, s);
     return SWIFT_OK;
@@ -623,22 +795,27 @@ int spmm_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dty
         set_error("SWIFT_KERNEL_ROWSORT is a column-major kernel");
         return SWIFT_ERR_UNSUPPORTED;
     }
-    const size_t csize = (dtype == SWIFT_F64) ? 8 : 4;
+    if (kernel == SWIFT_KERNEL_ROWVIEW && layout == SWIFT_COL_MAJOR) {
+        set_error("SWIFT_KERNEL_ROWVIEW is a row-major kernel");
+        return SWIFT_ERR_UNSUPPORTED;
+    }
     mark(0, s);
-    if (!(flags & SWIFT_ACCUMULATE))
-        SWIFT_TRY(fill_bytes_zero(C, (size_t)M->m * K * csize, s));
-    mark(1, s);
     const void *sv = lowp ? M->sortval_lp : M->sortval;
     const void *rv = lowp ? M->reside_val_lp : M->reside_val;
+    const void *vv = lowp ? M->rv_val_lp : M->rv_val;
     const float gs = M->lp_scale;
     if (layout == SWIFT_ROW_MAJOR) {
         switch (dtype) {
-            case SWIFT_F32: return spmm_row<float, float, float>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
-            case SWIFT_F64: return spmm_row<double, double, double>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
-            case SWIFT_BF16: return spmm_row<__hip_bfloat16, __hip_bfloat16, float>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
-            case SWIFT_F16: return spmm_row<__half, __half, float>(M, K, kernel, flags, B, C, sv, rv, kscale, gs, s);
+            case SWIFT_F32: return spmm_row<float, float, float>(M, K, kernel, flags, B, C, sv, rv, vv, nullptr, 1.f, s);
+            case SWIFT_F64: return spmm_row<double, double, double>(M, K, kernel, flags, B, C, sv, rv, vv, nullptr, 1.f, s);
+            case SWIFT_BF16: return spmm_row<__hip_bfloat16, __hip_bfloat16, float>(M, K, kernel, flags, B, C, sv, rv, vv, nullptr, 1.f, s);
+            case SWIFT_F16: return spmm_row<__half, __half, float>(M, K, kernel, flags, B, C, sv, rv, vv, kscale, gs, s);
         }
-    } else {  // the row-sorted kernel applies the scale in its run sums
+    } else {
+        const size_t csize = (dtype == SWIFT_F64) ? 8 : 4;
+        if (!(flags & SWIFT_ACCUMULATE))
+            SWIFT_TRY(fill_bytes_zero(C, (size_t)M->m * K * csize, s));
+        mark(1, s);
         switch (dtype) {
             case SWIFT_F32: return spmm_col<float, float, float>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
             case SWIFT_F64: return spmm_col<double, double, double>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
@@ -763,10 +940,11 @@ int prepare_values_impl(swift_matrix *M, swift_dtype dtype, hipStream_t s) {
         set_error("swift_prepare_values: dtype must be BF16 or F16");
         return SWIFT_ERR_ARG;
     }
-    if (M->sortval_lp) { (void)hipFree(M->sortval_lp); M->sortval_lp = nullptr; }
-    if (M->reside_val_lp) { (void)hipFree(M->reside_val_lp); M->reside_val_lp = nullptr; }
+    for (void **p : {&M->sortval_lp, &M->reside_val_lp, &M->rv_val_lp})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
     SWIFT_HIP(hipMalloc(&M->sortval_lp, std::max<size_t>(2, (size_t)M->regular_nnz * 2)));
     SWIFT_HIP(hipMalloc(&M->reside_val_lp, std::max<size_t>(2, (size_t)M->reside_nnz * 2)));
+    SWIFT_HIP(hipMalloc(&M->rv_val_lp, std::max<size_t>(2, (size_t)M->regular_nnz * 2)));
     float scale = 1.0f;
     if (dtype == SWIFT_F16) {
         Buffer mx;
@@ -806,6 +984,7 @@ int prepare_values_impl(swift_matrix *M, swift_dtype dtype, hipStream_t s) {
     };
     SWIFT_TRY(conv(M->sortval, M->sortval_lp, (size_t)M->regular_nnz));
     SWIFT_TRY(conv(M->reside_val, M->reside_val_lp, (size_t)M->reside_nnz));
+    SWIFT_TRY(conv(M->rv_val, M->rv_val_lp, (size_t)M->regular_nnz));
     SWIFT_HIP(hipStreamSynchronize(s));
     M->lp_type = dtype;
     M->lp_scale = scale;

@@ -142,10 +142,24 @@ struct swift_matrix {
     swift_dtype lp_type = SWIFT_F32;
     float lp_scale = 1.0f;
 
-    // column-tile work list for the regular kernels: (tile start, count)
+    // column-tile work list for the tile-order (scatter) kernels
     int nwork = 0;
     int2 *work = nullptr;        // (first tile, tile count) per work item
     int work_depth = 0;          // max tiles per work item
 
-    swift::Buffer scratch;       // reusable device workspace
+    // Row view of the regular part: the regular tiles' entries regrouped by
+    // output row (stable, so Swift order is kept inside a row), for the
+    // output-stationary kernels.
+    int *rv_ptr = nullptr;       // m+1
+    int *rv_col = nullptr;       // B row (original column) of each entry
+    void *rv_val = nullptr;      // value (vtype)
+    int *rv_src = nullptr;       // CSR position of each entry
+    void *rv_val_lp = nullptr;   // BF16/F16 copy (swift_prepare_values)
+    int *inv_perm = nullptr;     // inv_perm[perm[i]] = i
+    // merge-path split of (rows + regular entries) into segments of mp_items
+    int mp_items = 0, nseg = 0, nsplit = 0;
+    int2 *seg = nullptr;         // nseg+1 (row, entry) segment starts
+    int *split_rows = nullptr;   // rows whose entries span a segment boundary
+
+    mutable swift::Buffer scratch;  // reusable device workspace
 };

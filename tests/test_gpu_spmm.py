@@ -55,13 +55,16 @@ def test_col_major(swift, oracle, cuda, name, K, dt):
     assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
 
 
-@pytest.mark.parametrize("kernel", ["BALANCED", "UNBALANCED", "KCHUNK"])
+@pytest.mark.parametrize("kernel", ["BALANCED", "UNBALANCED", "KCHUNK", "ROWVIEW"])
 @pytest.mark.parametrize("layout", ["ROW", "COL"])
-@pytest.mark.parametrize("name", ["densecol_400x100", "equalrows_100x70", "powerlaw_3000x2500"])
+@pytest.mark.parametrize("name", ["densecol_400x100", "equalrows_100x70", "powerlaw_3000x2500",
+                                  "empty_10x40", "uniform_64x64"])
 def test_kernel_variants(swift, oracle, cuda, kernel, layout, name):
     print("case", name, kernel, layout, flush=True)
     if layout == "ROW" and kernel == "KCHUNK":
         pytest.skip("K chunks are a column-major grid dimension")
+    if layout == "COL" and kernel == "ROWVIEW":
+        pytest.skip("the row view is a row-major kernel")
     k = getattr(swift, "KERNEL_" + kernel)
     lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
     C, C64, absC = run_case(swift, oracle, name, 64, "f32", lay, k)
@@ -109,16 +112,19 @@ def test_b_sorted_flag(swift, oracle, cuda, layout, dt):
     assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
 
 
-def test_accumulate_flag(swift, oracle, cuda):
+@pytest.mark.parametrize("kernel", ["DEFAULT", "BALANCED"])
+@pytest.mark.parametrize("K", [32, 100])
+def test_accumulate_flag(swift, oracle, cuda, kernel, K):
     import torch
-    m, n, rp, ci, v = CASES["powerlaw_500x333"]
+    m, n, rp, ci, v = CASES["powerlaw_3000x2500"]
     vv = np.asarray(v, np.float32)
-    B = np.random.default_rng(1).uniform(-1, 1, (n, 32)).astype(np.float32)
-    C0 = np.random.default_rng(2).uniform(-1, 1, (m, 32)).astype(np.float32)
+    B = np.random.default_rng(1).uniform(-1, 1, (n, K)).astype(np.float32)
+    C0 = np.random.default_rng(2).uniform(-1, 1, (m, K)).astype(np.float32)
     C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
     M = swift.SwiftMatrix(m, n, rp, ci, vv)
     out = torch.tensor(C0, device="cuda")
-    M.spmm(torch.tensor(B, device="cuda"), out=out, flags=swift.ACCUMULATE)
+    M.spmm(torch.tensor(B, device="cuda"), out=out, flags=swift.ACCUMULATE,
+           kernel=getattr(swift, "KERNEL_" + kernel))
     got = out.cpu().numpy().astype(np.float64)
     assert gpu_common.rel_err(got - C0, C64, absC + np.abs(C0)) <= 1e-5
     M.close()
@@ -166,6 +172,32 @@ def test_host_launcher(swift, oracle, cuda):
                            sB.ctypes.data, out.ctypes.data, bad.ctypes.data, 0, 1,
                            C.byref(ms), C.byref(gf))
     assert rc == -14 and ms.value == -1.0
+    M.close()
+
+
+def test_rowview_col_major_rejected(swift, cuda):
+    import torch
+    m, n, rp, ci, v = CASES["random_300x200"]
+    M = swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float32)
+    with pytest.raises(swift.SwiftError):
+        M.spmm(torch.zeros((32, n), device="cuda"), layout=swift.COL_MAJOR,
+               kernel=swift.KERNEL_ROWVIEW)
+    M.close()
+
+
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "tall_2000x64"])
+def test_output_has_no_stale_values(swift, oracle, cuda, name):
+    """C = A*B overwrites every row (the row view stores rows, no zero pass):
+    fill C with garbage first."""
+    import torch
+    m, n, rp, ci, v = CASES[name]
+    vv = np.asarray(v, np.float32)
+    B = np.random.default_rng(5).uniform(-1, 1, (n, 128)).astype(np.float32)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv)
+    out = torch.full((m, 128), 1e30, device="cuda")
+    M.spmm(torch.tensor(B, device="cuda"), out=out)
+    assert gpu_common.rel_err(out.cpu().numpy(), C64, absC) <= 1e-5
     M.close()
 
 
