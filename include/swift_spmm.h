@@ -236,10 +236,11 @@ int swift_build_transpose(const swift_matrix *A, swift_matrix **AT,
 int swift_sddmm(const swift_matrix *A, int K, swift_dtype dtype,
                 const void *d_dC, const void *d_B, void *d_dvals,
                 swift_stream_t stream);
-/* Fused backward over A's Swift layout in one pass: dB = A^T*dC and
- * dvals = SDDMM(dC, B); dC is read once per nonzero.  Either output may be
- * NULL.  Row-major, original column order, F32/F64. */
-int swift_backward_fused(const swift_matrix *A, int K, swift_dtype dtype,
+/* Fused backward in one pass over the transposed Swift layout AT (from
+ * swift_build_transpose): dB = A^T*dC (n x K) and dvals = SDDMM(dC, B) in
+ * A's CSR order; every dC row is gathered once per nonzero and feeds both.
+ * Either output may be NULL.  Row-major, original column order, F32/F64. */
+int swift_backward_fused(const swift_matrix *AT, int K, swift_dtype dtype,
                          const void *d_dC, const void *d_B, void *d_dB,
                          void *d_dvals, swift_stream_t stream);
 /* Scatter new CSR-order values into the built format (values change, pattern

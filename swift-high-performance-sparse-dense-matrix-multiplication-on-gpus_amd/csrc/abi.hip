@@ -34,8 +34,10 @@ int prepare_values_impl(swift_matrix *M, swift_dtype dtype, hipStream_t s);
 int convert_dense_impl(const float *in, void *out, size_t rows, int K, swift_layout layout,
                        swift_dtype dtype, float *col_scale, hipStream_t s);
 int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t s);
-int backward_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, const void *B,
+int backward_impl(const swift_matrix *AT, int K, swift_dtype dtype, const void *dC, const void *B,
                   void *dB, void *dvals, hipStream_t s);
+int sddmm_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, const void *B,
+               void *dvals, hipStream_t s);
 int update_values_impl(swift_matrix *M, const void *csr, hipStream_t s);
 }  // namespace swift
 
@@ -336,13 +338,13 @@ extern "C" int swift_sddmm(const swift_matrix *A, int K, swift_dtype dtype, cons
         set_error("swift_sddmm: bad arguments");
         return SWIFT_ERR_ARG;
     }
-    SWIFT_GUARD(return backward_impl(A, K, dtype, d_dC, d_B, nullptr, d_dvals, S(stream));)
+    SWIFT_GUARD(return sddmm_impl(A, K, dtype, d_dC, d_B, d_dvals, S(stream));)
 }
 
-extern "C" int swift_backward_fused(const swift_matrix *A, int K, swift_dtype dtype,
+extern "C" int swift_backward_fused(const swift_matrix *AT, int K, swift_dtype dtype,
                                     const void *d_dC, const void *d_B, void *d_dB, void *d_dvals,
                                     swift_stream_t stream) {
-    SWIFT_GUARD(return backward_impl(A, K, dtype, d_dC, d_B, d_dB, d_dvals, S(stream));)
+    SWIFT_GUARD(return backward_impl(AT, K, dtype, d_dC, d_B, d_dB, d_dvals, S(stream));)
 }
 
 extern "C" int swift_update_values(swift_matrix *mat, const void *d_csr_val,

@@ -4,13 +4,13 @@
 //                          fused kernel below)
 //   dvals = SDDMM(dC, B)  dvals[e] = <dC[row_e,:], B[col_e,:]> on A's pattern
 //
-// The fused kernel walks A's own column tiles: the 32 columns of a work item
-// are fixed across its depth tiles, so dB rows for those columns accumulate in
-// registers and are flushed with one atomic per column per work item, while
-// each entry's dC row is read once and reused for its dvals dot product.
+// The fused pass walks the transposed Swift layout (built once by
+// swift_build_transpose): each dC row is gathered once per nonzero and feeds
+// both dB and dvals, and dB rows are written once, not scattered.
 #include "swift_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace swift {
 
@@ -130,6 +130,9 @@ int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t 
     if (T->nnz)
         hipLaunchKernelGGL(k_compose, dim3(g1(T->nnz)), dim3(256), 0, s, T->sortsrc_tmp,
                            cscsrc.as<int>(), (size_t)T->nnz);
+    if (T->regular_nnz)
+        hipLaunchKernelGGL(k_compose, dim3(g1(T->regular_nnz)), dim3(256), 0, s, T->rv_src,
+                           cscsrc.as<int>(), (size_t)T->regular_nnz);
     SWIFT_CHECK_LAUNCH();
     SWIFT_HIP(hipStreamSynchronize(s));
     *out = T;
@@ -137,40 +140,73 @@ int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t 
 }
 
 // ------------------------------------------------------- SDDMM / fused --
+//
+// Both passes walk a row view with merge-path segments (one wave per
+// segment, EPS sub-groups of LPE lanes, lanes along K), like the forward
+// row-view kernel:
+//   SDDMM over A:    row i keeps H = dC[i] in registers; each entry (j, src)
+//                    gathers G = B[j]; dvals[src] = <H, G>.
+//   fused over A^T:  row j of A^T (= column j of A) keeps H = B[j]; each entry
+//                    (i, v, src) gathers G = dC[i] once and uses it twice:
+//                    dvals[src] = <H, G> and dB[j] += v * G.
+// The walked matrix's reside blocks (one column, <= 32 entries) follow with
+// the roles swapped: the column operand is loaded once, dB takes atomics.
 
 struct BwArgs {
-    int nwork;
-    const int2 *work;
-    const int *tile_colidx, *srow, *ssrc;
-    const void *sval;
-    int nblk;
+    int m, nseg;               // walked matrix rows, merge-path segments
+    const int2 *seg;
+    const int *rv_ptr, *rv_col, *rv_src;
+    const void *rv_val;
+    int n, nblk;               // walked matrix columns, reside blocks
     const unsigned *colorblk;
     const int *blkstart, *blkstop, *rptr, *rrow, *rsrc;
     const void *rval;
-    int n, K;
     const int *perm;
-    const void *dC, *B;
-    void *dB, *dvals;
+    int K;
+    const void *G;             // operand gathered per entry (by column)
+    const void *H;             // operand per walked row
+    void *dB;                  // walked-row output (fused pass) or nullptr
+    void *dvals;               // CSR-order output or nullptr
+    const int *split_rows;
+    int nsplit;
 };
 
 template <class T> struct V16;
-template <> struct V16<float> { using t = float4; static constexpr int N = 4; };
-template <> struct V16<double> { using t = double2; static constexpr int N = 2; };
+template <> struct V16<float> { static constexpr int N = 4; };
+template <> struct V16<double> { static constexpr int N = 2; };
 
 template <class T, int VEC>
-__device__ __forceinline__ void ld(const T *p, T (&o)[VEC]) {
-    if constexpr (VEC == 4) {
+__device__ __forceinline__ void ldv(const T *p, T (&o)[VEC]) {
+    if constexpr (std::is_same<T, float>::value && VEC == 4) {
         float4 v = *reinterpret_cast<const float4 *>(p);
         o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-    } else if constexpr (VEC == 2) {
+    } else if constexpr (std::is_same<T, double>::value && VEC == 2) {
         double2 v = *reinterpret_cast<const double2 *>(p);
         o[0] = v.x; o[1] = v.y;
     } else {
-        o[0] = p[0];
+#pragma unroll
+        for (int i = 0; i < VEC; i++) o[i] = p[i];
     }
 }
 
-// Sum over the LPE lanes of one entry group.
+template <class T, int VEC>
+__device__ __forceinline__ void stv(T *p, const T (&x)[VEC]) {
+    if constexpr (std::is_same<T, float>::value && VEC == 4) {
+        *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
+    } else if constexpr (std::is_same<T, double>::value && VEC == 2) {
+        *reinterpret_cast<double2 *>(p) = make_double2(x[0], x[1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < VEC; i++) p[i] = x[i];
+    }
+}
+
+__device__ __forceinline__ unsigned xcd_block_bw(unsigned b, unsigned nblocks) {
+    const unsigned per = nblocks / 8, rem = nblocks % 8, x = b % 8;
+    return x * per + min(x, rem) + b / 8;
+}
+
+// Sum over the LPE lanes of one sub-group.
 template <class T, int LPE>
 __device__ __forceinline__ T group_sum(T v) {
 #pragma unroll
@@ -178,81 +214,124 @@ __device__ __forceinline__ T group_sum(T v) {
     return v;
 }
 
-// One wave per work item; lanes along K (LPE lanes x VEC x Q elements).
-// MODE bit 0: dvals, bit 1: dB.
-template <class T, int VEC, int Q, int LPE, int MODE>
-__global__ __launch_bounds__(256) void k_bw_regular(BwArgs a) {
-    constexpr int EPS = 64 / LPE, STEPS = TILE / EPS;
+template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
+__global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
+    constexpr int EPS = 64 / LPE;
+    constexpr int UNROLL = 2;
+    constexpr int STEP = EPS * UNROLL;
     const int lane = threadIdx.x & 63;
     const int h = lane / LPE, sub = lane % LPE;
-    const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (w >= a.nwork) return;
-    const int2 wk = a.work[w];
-    const int c0 = a.tile_colidx[wk.x];
-    const int my = lane & 31;
-    const int mycol = a.perm[c0 + my];
-    const T *dC = static_cast<const T *>(a.dC);
-    const T *B = static_cast<const T *>(a.B);
-    const T *sval = static_cast<const T *>(a.sval);
-    T db[STEPS][Q][VEC];
-    if constexpr (MODE & 2) {
+    const unsigned lb = xcd_block_bw(blockIdx.x, gridDim.x);
+    const int sid = (int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (sid >= a.nseg) return;
+    const int2 st = a.seg[sid], en = a.seg[sid + 1];
+    const int r1 = en.x, e1 = en.y;
+    int row = st.x, e = st.y;
+    const bool carried_in = row < a.m && e > a.rv_ptr[row];
+    const T *val = static_cast<const T *>(a.rv_val);
+    const T *G = static_cast<const T *>(a.G);
+    const T *H = static_cast<const T *>(a.H);
+    T acc[Q][VEC], hr[Q][VEC];
 #pragma unroll
-        for (int i = 0; i < STEPS; i++)
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[q][v] = (T)0;
+    while (true) {
+        const int row_end = row < r1 ? a.rv_ptr[row + 1] : e1;
+        if (DV && e < row_end) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) ldv<T, VEC>(H + (size_t)row * a.K + q * LPE * VEC + sub * VEC, hr[q]);
+        }
+        while (e < row_end) {
+            const int nb = min(64, row_end - e);
+            int mc = 0, ms = 0;
+            T mv = (T)0;
+            if (lane < nb) {
+                mc = a.rv_col[e + lane];
+                ms = a.rv_src[e + lane];
+                mv = val[e + lane];
+            }
+            for (int j = 0; j < nb; j += STEP) {
+                T g[UNROLL][Q][VEC];
+                T vv[UNROLL];
+                int src[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++) {
+                    const int slot = j + u * EPS + h;
+                    // unconditional shuffles: see the forward row-view kernel
+                    const int c = __shfl(mc, slot & 63, 64);
+                    src[u] = __shfl(ms, slot & 63, 64);
+                    const T vs = __shfl(mv, slot & 63, 64);
+                    vv[u] = slot < nb ? vs : (T)0;
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        if (slot < nb)
+                            ldv<T, VEC>(G + (size_t)c * a.K + q * LPE * VEC + sub * VEC, g[u][q]);
+                        else
+#pragma unroll
+                            for (int v = 0; v < VEC; v++) g[u][q][v] = (T)0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++) {
+                    if (DB) {
+#pragma unroll
+                        for (int q = 0; q < Q; q++)
+#pragma unroll
+                            for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * g[u][q][v];
+                    }
+                    if (DV) {
+                        T d = (T)0;
+#pragma unroll
+                        for (int q = 0; q < Q; q++)
+#pragma unroll
+                            for (int v = 0; v < VEC; v++) d += g[u][q][v] * hr[q][v];
+                        d = group_sum<T, LPE>(d);
+                        if (sub == 0 && j + u * EPS + h < nb) static_cast<T *>(a.dvals)[src[u]] = d;
+                    }
+                }
+            }
+            e += nb;
+        }
+        const bool last = row >= r1;
+        if (!DB) {
+            if (last) break;
+            row++;
+            continue;
+        }
+        if (last && !(r1 < a.m && e1 > a.rv_ptr[r1])) break;
+#pragma unroll
+        for (int off = LPE; off < 64; off <<= 1)
 #pragma unroll
             for (int q = 0; q < Q; q++)
 #pragma unroll
-                for (int v = 0; v < VEC; v++) db[i][q][v] = (T)0;
-    }
-    for (int t = wk.x; t < wk.x + wk.y; t++) {
-        const size_t e0 = (size_t)t * TILE;
-        const int myrow = a.srow[e0 + my];
-        const T myval = sval[e0 + my];
-        const int mysrc = a.ssrc[e0 + my];
-#pragma unroll
-        for (int i = 0; i < STEPS; i++) {
-            const int l = h + i * EPS;
-            const int r = __shfl(myrow, l, 64);
-            const T v = __shfl(myval, l, 64);
-            const int bc = __shfl(mycol, l, 64);
-            T dot = (T)0;
+                for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], off, 64);
+        if (h == 0) {
+            T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K;
+            const bool split = last || (row == st.x && carried_in);
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const int k0 = q * LPE * VEC + sub * VEC;
-                T g[VEC];
-                ld<T, VEC>(dC + (size_t)r * a.K + k0, g);
-                if constexpr (MODE & 1) {
-                    T b[VEC];
-                    ld<T, VEC>(B + (size_t)bc * a.K + k0, b);
+                if (split) {
 #pragma unroll
-                    for (int x = 0; x < VEC; x++) dot += g[x] * b[x];
-                }
-                if constexpr (MODE & 2) {
-#pragma unroll
-                    for (int x = 0; x < VEC; x++) db[i][q][x] += v * g[x];
+                    for (int v = 0; v < VEC; v++) atomic_add_bw(drow + k0 + v, acc[q][v]);
+                } else {
+                    stv<T, VEC>(drow + k0, acc[q]);
                 }
             }
-            if constexpr (MODE & 1) {
-                dot = group_sum<T, LPE>(dot);
-                const int src = __shfl(mysrc, l, 64);
-                if (sub == 0) static_cast<T *>(a.dvals)[src] = dot;
-            }
         }
-    }
-    if constexpr (MODE & 2) {
-        T *dB = static_cast<T *>(a.dB);
 #pragma unroll
-        for (int i = 0; i < STEPS; i++) {
-            const int bc = __shfl(mycol, h + i * EPS, 64);
+        for (int q = 0; q < Q; q++)
 #pragma unroll
-            for (int q = 0; q < Q; q++)
-#pragma unroll
-                for (int x = 0; x < VEC; x++)
-                    atomic_add_bw(dB + (size_t)bc * a.K + q * LPE * VEC + sub * VEC + x, db[i][q][x]);
-        }
+            for (int v = 0; v < VEC; v++) acc[q][v] = (T)0;
+        if (last) break;
+        row++;
     }
 }
 
-template <class T, int VEC, int Q, int LPE, int MODE>
+// Reside blocks of the walked matrix: column operand G[c] loaded once, each
+// entry gathers H[row]; dvals = <H[row], G[c]>, dB[row] += v * G[c].
+template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
 __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
     constexpr int EPS = 64 / LPE;
     const int lane = threadIdx.x & 63;
@@ -264,71 +343,68 @@ __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
     const bool split = (u >> 31) != 0;
     const int start = split ? a.blkstart[blk] : a.rptr[r];
     const int stop = split ? a.blkstop[blk] : a.rptr[r + 1];
-    const int bc = a.perm[a.n - 1 - r];
-    const T *dC = static_cast<const T *>(a.dC);
-    const T *B = static_cast<const T *>(a.B);
+    const int c = a.perm[a.n - 1 - r];
+    const T *G = static_cast<const T *>(a.G);
+    const T *H = static_cast<const T *>(a.H);
     const T *rval = static_cast<const T *>(a.rval);
-    T b[Q][VEC], db[Q][VEC];
+    T gc[Q][VEC];
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-        ld<T, VEC>(B + (size_t)bc * a.K + q * LPE * VEC + sub * VEC, b[q]);
-#pragma unroll
-        for (int x = 0; x < VEC; x++) db[q][x] = (T)0;
-    }
+    for (int q = 0; q < Q; q++) ldv<T, VEC>(G + (size_t)c * a.K + q * LPE * VEC + sub * VEC, gc[q]);
     for (int e0 = start; e0 < stop; e0 += EPS) {
         const int e = e0 + h;
         const bool ok = e < stop;
         const int row = ok ? a.rrow[e] : 0;
-        const T v = ok ? rval[e] : (T)0;
-        T dot = (T)0;
+        if (DV) {
+            T d = (T)0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) {
-            T g[VEC];
-            if (ok) ld<T, VEC>(dC + (size_t)row * a.K + q * LPE * VEC + sub * VEC, g);
-            else
+            for (int q = 0; q < Q; q++) {
+                T hv[VEC];
+                if (ok) ldv<T, VEC>(H + (size_t)row * a.K + q * LPE * VEC + sub * VEC, hv);
+                else
 #pragma unroll
-                for (int x = 0; x < VEC; x++) g[x] = (T)0;
+                    for (int v = 0; v < VEC; v++) hv[v] = (T)0;
 #pragma unroll
-            for (int x = 0; x < VEC; x++) {
-                dot += g[x] * b[q][x];
-                db[q][x] += v * g[x];
+                for (int v = 0; v < VEC; v++) d += hv[v] * gc[q][v];
             }
+            d = group_sum<T, LPE>(d);
+            if (ok && sub == 0) static_cast<T *>(a.dvals)[a.rsrc[e]] = d;
         }
-        if constexpr (MODE & 1) {
-            dot = group_sum<T, LPE>(dot);
-            if (ok && sub == 0) static_cast<T *>(a.dvals)[a.rsrc[e]] = dot;
+        if (DB && ok) {
+            const T v = rval[e];
+            T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K;
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int x = 0; x < VEC; x++)
+                    atomic_add_bw(drow + q * LPE * VEC + sub * VEC + x, v * gc[q][x]);
         }
-    }
-    if constexpr (MODE & 2) {
-        // the EPS groups hold partial sums of the same B row
-#pragma unroll
-        for (int q = 0; q < Q; q++)
-#pragma unroll
-            for (int x = 0; x < VEC; x++) {
-                T v = db[q][x];
-#pragma unroll
-                for (int off = LPE; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-                if (h == 0)
-                    atomic_add_bw(static_cast<T *>(a.dB) + (size_t)bc * a.K + q * LPE * VEC +
-                                      sub * VEC + x,
-                                  v);
-            }
     }
 }
 
-template <class T, int VEC, int Q, int LPE, int MODE>
+template <class T>
+__global__ void k_zero_rows_bw(const int *__restrict__ rows, int nrows, T *__restrict__ D, int K) {
+    const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (i >= nrows) return;
+    T *d = D + (size_t)rows[i] * K;
+    for (int k = threadIdx.x & 63; k < K; k += 64) d[k] = (T)0;
+}
+
+template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
 static int launch_bw(const BwArgs &a, hipStream_t s) {
-    if (a.nwork)
-        hipLaunchKernelGGL((k_bw_regular<T, VEC, Q, LPE, MODE>),
-                           dim3(ceil_div((size_t)a.nwork * 64, 256)), dim3(256), 0, s, a);
+    if (DB && a.nsplit)
+        hipLaunchKernelGGL(k_zero_rows_bw<T>, dim3(ceil_div((size_t)a.nsplit * 64, 256)), dim3(256),
+                           0, s, a.split_rows, a.nsplit, (T *)a.dB, a.K);
+    if (a.nseg)
+        hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, DB, DV>),
+                           dim3(ceil_div((size_t)a.nseg * 64, 256)), dim3(256), 0, s, a);
     if (a.nblk)
-        hipLaunchKernelGGL((k_bw_reside<T, VEC, Q, LPE, MODE>),
+        hipLaunchKernelGGL((k_bw_reside<T, VEC, Q, LPE, DB, DV>),
                            dim3(ceil_div((size_t)a.nblk * 64, 256)), dim3(256), 0, s, a);
     SWIFT_CHECK_LAUNCH();
     return SWIFT_OK;
 }
 
-template <class T, int MODE>
+template <class T, bool DB, bool DV>
 static int dispatch_bw(const BwArgs &a, hipStream_t s) {
     constexpr int V = V16<T>::N;
     const int K = a.K;
@@ -336,62 +412,76 @@ static int dispatch_bw(const BwArgs &a, hipStream_t s) {
         set_error("backward kernels need K %% %d == 0 (K=%d)", V, K);
         return SWIFT_ERR_UNSUPPORTED;
     }
-    const int lanes = K / V;  // lanes x Q
-    switch (lanes) {
-        case 8: return launch_bw<T, V, 1, 8, MODE>(a, s);
-        case 16: return launch_bw<T, V, 1, 16, MODE>(a, s);
-        case 32: return launch_bw<T, V, 1, 32, MODE>(a, s);
-        case 64: return launch_bw<T, V, 1, 64, MODE>(a, s);
-        case 128: return launch_bw<T, V, 2, 64, MODE>(a, s);
+    switch (K / V) {
+        case 8: return launch_bw<T, V, 1, 8, DB, DV>(a, s);
+        case 16: return launch_bw<T, V, 1, 16, DB, DV>(a, s);
+        case 32: return launch_bw<T, V, 1, 32, DB, DV>(a, s);
+        case 64: return launch_bw<T, V, 1, 64, DB, DV>(a, s);
+        case 128: return launch_bw<T, V, 2, 64, DB, DV>(a, s);
+        case 256: return launch_bw<T, V, 4, 64, DB, DV>(a, s);
         default: break;
     }
-    set_error("backward kernels support K/%d in {8,16,32,64,128} (K=%d)", V, K);
+    set_error("backward kernels support K/%d in {8,16,32,64,128,256} (K=%d)", V, K);
     return SWIFT_ERR_UNSUPPORTED;
 }
 
-int backward_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, const void *B,
-                  void *dB, void *dvals, hipStream_t s) {
-    if (!A || K <= 0 || !dC || (!dB && !dvals) || (dvals && !B)) {
-        set_error("swift_backward: bad arguments");
-        return SWIFT_ERR_ARG;
-    }
-    if (dtype != A->vtype) {
-        set_error("swift_backward: dtype must equal the build value type");
-        return SWIFT_ERR_ARG;
-    }
-    const size_t es = dtype == SWIFT_F64 ? 8 : 4;
-    if (dB) SWIFT_TRY(fill_bytes_zero(dB, (size_t)A->n * K * es, s));
+static BwArgs walk_args(const swift_matrix *W, int K) {
     BwArgs a{};
-    a.nwork = A->nwork;
-    a.work = A->work;
-    a.tile_colidx = A->tile_colidx;
-    a.srow = A->sortrowidx;
-    a.ssrc = A->sortsrc;
-    a.sval = A->sortval;
-    a.nblk = A->numblk;
-    a.colorblk = A->colorblk;
-    a.blkstart = A->blkstart;
-    a.blkstop = A->blkstop;
-    a.rptr = A->reside_cscptr;
-    a.rrow = A->reside_cscrowidx;
-    a.rsrc = A->reside_src;
-    a.rval = A->reside_val;
-    a.n = A->n;
+    a.m = W->m;
+    a.nseg = W->nseg;
+    a.seg = W->seg;
+    a.rv_ptr = W->rv_ptr;
+    a.rv_col = W->rv_col;
+    a.rv_src = W->rv_src;
+    a.rv_val = W->rv_val;
+    a.n = W->n;
+    a.nblk = W->numblk;
+    a.colorblk = W->colorblk;
+    a.blkstart = W->blkstart;
+    a.blkstop = W->blkstop;
+    a.rptr = W->reside_cscptr;
+    a.rrow = W->reside_cscrowidx;
+    a.rsrc = W->reside_src;
+    a.rval = W->reside_val;
+    a.perm = W->perm;
+    a.split_rows = W->split_rows;
+    a.nsplit = W->nsplit;
     a.K = K;
-    a.perm = A->perm;
-    a.dC = dC;
-    a.B = B;
+    return a;
+}
+
+// dvals = SDDMM(dC, B) walking A: H = dC (per row), G = B (per entry).
+int sddmm_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, const void *B,
+               void *dvals, hipStream_t s) {
+    if (!A || K <= 0 || !dC || !B || !dvals || dtype != A->vtype) {
+        set_error("swift_sddmm: bad arguments (dtype must equal the build value type)");
+        return SWIFT_ERR_ARG;
+    }
+    BwArgs a = walk_args(A, K);
+    a.H = dC;
+    a.G = B;
+    a.dvals = dvals;
+    return dtype == SWIFT_F64 ? dispatch_bw<double, false, true>(a, s)
+                              : dispatch_bw<float, false, true>(a, s);
+}
+
+// Fused backward walking A^T: H = B (per A^T row = A column), G = dC (per
+// entry, gathered once), dB rows are A^T's rows.
+int backward_impl(const swift_matrix *AT, int K, swift_dtype dtype, const void *dC, const void *B,
+                  void *dB, void *dvals, hipStream_t s) {
+    if (!AT || K <= 0 || !dC || (!dB && !dvals) || (dvals && !B) || dtype != AT->vtype) {
+        set_error("swift_backward_fused: bad arguments (dtype must equal the build value type)");
+        return SWIFT_ERR_ARG;
+    }
+    BwArgs a = walk_args(AT, K);
+    a.H = B;
+    a.G = dC;
     a.dB = dB;
     a.dvals = dvals;
-    const int mode = (dvals ? 1 : 0) | (dB ? 2 : 0);
-    if (dtype == SWIFT_F64) {
-        if (mode == 1) return dispatch_bw<double, 1>(a, s);
-        if (mode == 2) return dispatch_bw<double, 2>(a, s);
-        return dispatch_bw<double, 3>(a, s);
-    }
-    if (mode == 1) return dispatch_bw<float, 1>(a, s);
-    if (mode == 2) return dispatch_bw<float, 2>(a, s);
-    return dispatch_bw<float, 3>(a, s);
+    const bool f64 = dtype == SWIFT_F64;
+    if (dB && dvals) return f64 ? dispatch_bw<double, true, true>(a, s) : dispatch_bw<float, true, true>(a, s);
+    if (dB) return f64 ? dispatch_bw<double, true, false>(a, s) : dispatch_bw<float, true, false>(a, s);
+    return f64 ? dispatch_bw<double, false, true>(a, s) : dispatch_bw<float, false, true>(a, s);
 }
 
 }  // namespace swift

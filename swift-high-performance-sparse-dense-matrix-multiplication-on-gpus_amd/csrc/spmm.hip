@@ -288,18 +288,35 @@ __device__ __forceinline__ void put_row(const RowArgs &a, int row, int sub, bool
     }
 }
 
-// Merge-path walk of the row view: each group of LPE lanes owns one segment of
-// W items (rows + entries), so every group does the same amount of work
-// whatever the row lengths.  Rows finished inside the segment are stored;
-// rows cut by a segment boundary are split rows (zeroed beforehand) and take
-// atomic adds.  Entries are consumed four at a time so that four 16-byte B
-// row loads are in flight per lane.
+// Block order -> logical block such that each XCD (blocks are dealt to the 8
+// XCDs round-robin) gets one contiguous range of logical blocks: neighbouring
+// segments cover neighbouring rows, whose B rows then meet in one L2.  Any
+// other dispatch order only costs locality, never correctness.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks) {
+    constexpr unsigned X = 8;
+    const unsigned per = nblocks / X, rem = nblocks % X, x = b % X;
+    return x * per + min(x, rem) + b / X;
+}
+
+// Merge-path walk of the row view: each wave owns one segment of W items
+// (rows + entries), so every wave does the same amount of work whatever the
+// row lengths.  The segment's entries are contiguous in the row view, so they
+// are staged 64 at a time (column, value) together with 64 row ends, and rows
+// are walked inside the staged batch: the only dependent loads are the B row
+// gathers.  Lanes split into EPS sub-groups of LPE lanes; sub-group h takes
+// entries h, h+EPS, ... with UNROLL 16-byte loads per lane in flight; at a
+// row end the sub-groups' sums are combined with a butterfly and sub-group 0
+// stores the row.  Control flow is uniform across the wave.  Rows cut by a
+// segment boundary are split rows (zeroed beforehand) and take atomics.
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
 __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
-    constexpr int G = 64 / LPE;
+    constexpr int EPS = 64 / LPE;
+    constexpr int UNROLL = 4;
+    constexpr int STEP = EPS * UNROLL;  // entries per iteration
     const int lane = threadIdx.x & 63;
-    const int grp = lane / LPE, sub = lane % LPE;
-    const int sid = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6) * G + grp;
+    const int h = lane / LPE, sub = lane % LPE;
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int sid = (int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6);
     if (sid >= a.nseg) return;
     const int2 st = a.seg[sid], en = a.seg[sid + 1];
     const int r1 = en.x, e1 = en.y;
@@ -307,34 +324,58 @@ __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
     const bool carried_in = row < a.m && e > a.rv_ptr[row];
     const TV *val = static_cast<const TV *>(a.rv_val);
     const TB *B = static_cast<const TB *>(a.B);
+    // staged row ends: lane l holds the end of row rbase + l
+    int rbase = row;
+    int rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+    // staged entries: lane l holds entry ebase + l
+    int ebase = e, mc = 0;
+    TC mv = (TC)0;
+    if (ebase + lane < e1) {
+        const int c = a.rv_col[ebase + lane];
+        mc = a.inv_perm ? a.inv_perm[c] : c;
+        mv = to_acc(val[ebase + lane]);
+    }
     TC acc[Q][VEC];
 #pragma unroll
     for (int q = 0; q < Q; q++)
 #pragma unroll
         for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
     while (true) {
-        const int row_end = row < r1 ? a.rv_ptr[row + 1] : e1;
+        if (row - rbase >= 64) {
+            rbase = row;
+            rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+        }
+        const int rend = __shfl(rend_l, row - rbase, 64);
+        const int row_end = row < r1 ? rend : e1;
         while (e < row_end) {
-            const int nb = min(LPE, row_end - e);
-            int mc = 0;
-            TC mv = (TC)0;
-            if (sub < nb) {
-                const int c = a.rv_col[e + sub];
-                mc = a.inv_perm ? a.inv_perm[c] : c;
-                mv = to_acc(val[e + sub]);
+            if (e - ebase >= 64) {
+                ebase = e;
+                mc = 0;
+                mv = (TC)0;
+                if (ebase + lane < e1) {
+                    const int c = a.rv_col[ebase + lane];
+                    mc = a.inv_perm ? a.inv_perm[c] : c;
+                    mv = to_acc(val[ebase + lane]);
+                }
             }
-            for (int j = 0; j < nb; j += 4) {
-                TC b[4][Q][VEC];
-                TC vv[4];
+            const int off = e - ebase;
+            const int nb = min(64 - off, row_end - e);  // entries of this row in the batch
+            for (int j = 0; j < nb; j += STEP) {
+                TC b[UNROLL][Q][VEC];
+                TC vv[UNROLL];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int c = __shfl(mc, j + u, LPE);
-                    vv[u] = __shfl(mv, j + u, LPE);
+                for (int u = 0; u < UNROLL; u++) {
+                    const int idx = j + u * EPS + h;
+                    // every lane takes part in both shuffles (a lane reading a
+                    // source lane that is masked off would get a stale value)
+                    const int c = __shfl(mc, (off + idx) & 63, 64);
+                    const TC vs = __shfl(mv, (off + idx) & 63, 64);
+                    vv[u] = idx < nb ? vs : (TC)0;
                     const TB *brow = B + (size_t)c * a.ldb + a.kbase;
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
                         const int k0 = q * LPE * VEC + sub * VEC;
-                        if (j + u < nb && (!MASK || k0 < a.kcount))
+                        if (idx < nb && (!MASK || k0 < a.kcount))
                             load_b<TB, VEC, TC>(brow + k0, b[u][q]);
                         else
 #pragma unroll
@@ -342,7 +383,7 @@ __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++)
+                for (int u = 0; u < UNROLL; u++)
 #pragma unroll
                     for (int q = 0; q < Q; q++)
 #pragma unroll
@@ -350,11 +391,25 @@ __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
             }
             e += nb;
         }
-        if (row >= r1) break;
-        put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, row == st.x && carried_in, acc);
+        const bool last = row >= r1;
+        if (last && !(r1 < a.m && e1 > a.rv_ptr[r1])) break;
+        // combine the EPS sub-groups' partial sums of this row
+#pragma unroll
+        for (int o = LPE; o < 64; o <<= 1)
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
+        if (h == 0)
+            put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, last || (row == st.x && carried_in), acc);
+        else
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+        if (last) break;
         row++;
     }
-    if (r1 < a.m && e1 > a.rv_ptr[r1]) put_row<TC, VEC, Q, LPE, MASK>(a, r1, sub, true, acc);
 }
 
 // Zeroes C[row, kbase : kbase+kcount) of each split row; one wave per row.
@@ -576,11 +631,9 @@ enum RowKind { ROW_TILES = 0, ROW_RESIDE = 1, ROW_VIEW = 2 };
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
 static void launch_row_shape(const RowArgs &a, RowKind kind, hipStream_t s) {
     if (kind == ROW_VIEW) {
-        constexpr int G = 64 / LPE;
-        const size_t waves = ((size_t)a.nseg + G - 1) / G;
-        if (waves)
+        if (a.nseg)
             hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK>),
-                               dim3(ceil_div(waves * 64, 256)), dim3(256), 0, s, a);
+                               dim3(ceil_div((size_t)a.nseg * 64, 256)), dim3(256), 0, s, a);
         return;
     }
     const unsigned items = kind == ROW_TILES ? (unsigned)a.nwork : (unsigned)a.nblk;
@@ -687,10 +740,67 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     return SWIFT_OK;
 }
 
+// out[c * rows + r] (+)= in[r * cols + c]: 32x32 tiles staged through LDS so
+// both the read and the write are coalesced; one block per tile (1-D grid).
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose(const T *__restrict__ in, T *__restrict__ out,
+                                                   int rows, int cols, int tiles_c, int accumulate) {
+    __shared__ T tile[32][33];
+    const int tcol = (int)(blockIdx.x % (unsigned)tiles_c), trow = (int)(blockIdx.x / (unsigned)tiles_c);
+    const int c0 = tcol * 32, r0 = trow * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+#pragma unroll
+    for (int i = 0; i < 32; i += 8) {
+        const int r = r0 + ty + i, c = c0 + tx;
+        if (r < rows && c < cols) tile[ty + i][tx] = in[(size_t)r * cols + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 32; i += 8) {
+        const int c = c0 + ty + i, r = r0 + tx;
+        if (r < rows && c < cols) {
+            T *o = out + (size_t)c * rows + r;
+            *o = accumulate ? (T)(*o + tile[tx][ty + i]) : tile[tx][ty + i];
+        }
+    }
+}
+
+template <class T>
+static int transpose(const T *in, T *out, int rows, int cols, bool accumulate, hipStream_t s) {
+    if (rows == 0 || cols == 0) return SWIFT_OK;
+    const size_t tc = ceil_div(cols, 32), tr = ceil_div(rows, 32);
+    if (tc * tr > 0x7FFFFFFFull) {
+        set_error("transpose: %d x %d is too large", rows, cols);
+        return SWIFT_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(k_transpose<T>, dim3((unsigned)(tc * tr)), dim3(256), 0, s, in, out, rows,
+                       cols, (int)tc, accumulate ? 1 : 0);
+    SWIFT_CHECK_LAUNCH();
+    return SWIFT_OK;
+}
+
+// Column-major C = A*B.  DEFAULT: B is transposed into the matrix's
+// workspace, the row-view product runs row-major, and C is transposed back
+// (3 HBM-bound passes instead of strided scatter atomics).  Other kernels:
+// the reference's tile order with lanes along the 32 columns of a tile.
 template <class TV, class TB, class TC>
 static int spmm_col(const swift_matrix *M, int K, swift_kernel kernel, unsigned flags,
                     const void *B, void *C, const void *sval, const void *rval,
-                    const float *kscale, float gscale, hipStream_t s) {
+                    const void *rv_val, const float *kscale, float gscale, hipStream_t s) {
+    const bool accumulate = (flags & SWIFT_ACCUMULATE) != 0;
+    if (kernel == SWIFT_KERNEL_DEFAULT) {
+        const size_t bB = ((size_t)M->n * K * sizeof(TB) + 255) & ~(size_t)255;
+        const size_t bC = (size_t)M->m * K * sizeof(TC);
+        SWIFT_TRY(M->scratch.alloc(bB + bC + 256));
+        TB *Bt = M->scratch.as<TB>();
+        TC *Ct = reinterpret_cast<TC *>(static_cast<char *>(M->scratch.ptr) + bB);
+        SWIFT_TRY(transpose<TB>(static_cast<const TB *>(B), Bt, K, M->n, false, s));
+        SWIFT_TRY((spmm_row<TV, TB, TC>(M, K, SWIFT_KERNEL_ROWVIEW, flags & ~SWIFT_ACCUMULATE, Bt,
+                                        Ct, sval, rval, rv_val, kscale, gscale, s)));
+        return transpose<TC>(Ct, static_cast<TC *>(C), M->m, K, accumulate, s);
+    }
+    if (!accumulate) SWIFT_TRY(fill_bytes_zero(C, (size_t)M->m * K * sizeof(TC), s));
+    mark(1, s);
     ColArgs a{};
     a.tilenum = M->tilenum;
     a.tile_colidx = M->tile_colidx;
@@ -720,8 +830,7 @@ static int spmm_col(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.rowsorted = M->rowsorted;
     // K chunk per blockIdx.y: the reference's Swift_GPU_3/4 use 32-wide chunks
     // (src/Swift_3.h:338); the others walk all of K in one wave.
-    const bool chunked = kernel == SWIFT_KERNEL_KCHUNK || kernel == SWIFT_KERNEL_ROWSORT ||
-                         kernel == SWIFT_KERNEL_DEFAULT;
+    const bool chunked = kernel == SWIFT_KERNEL_KCHUNK || kernel == SWIFT_KERNEL_ROWSORT;
     a.kchunk = chunked ? std::min(K, 32) : K;
     const unsigned ky = (unsigned)((K + a.kchunk - 1) / a.kchunk);
     if (kernel == SWIFT_KERNEL_ROWSORT) {
@@ -812,15 +921,11 @@ int spmm_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dty
             case SWIFT_F16: return spmm_row<__half, __half, float>(M, K, kernel, flags, B, C, sv, rv, vv, kscale, gs, s);
         }
     } else {
-        const size_t csize = (dtype == SWIFT_F64) ? 8 : 4;
-        if (!(flags & SWIFT_ACCUMULATE))
-            SWIFT_TRY(fill_bytes_zero(C, (size_t)M->m * K * csize, s));
-        mark(1, s);
         switch (dtype) {
-            case SWIFT_F32: return spmm_col<float, float, float>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
-            case SWIFT_F64: return spmm_col<double, double, double>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
-            case SWIFT_BF16: return spmm_col<__hip_bfloat16, __hip_bfloat16, float>(M, K, kernel, flags, B, C, sv, rv, nullptr, 1.f, s);
-            case SWIFT_F16: return spmm_col<__half, __half, float>(M, K, kernel, flags, B, C, sv, rv, kscale, gs, s);
+            case SWIFT_F32: return spmm_col<float, float, float>(M, K, kernel, flags, B, C, sv, rv, vv, nullptr, 1.f, s);
+            case SWIFT_F64: return spmm_col<double, double, double>(M, K, kernel, flags, B, C, sv, rv, vv, nullptr, 1.f, s);
+            case SWIFT_BF16: return spmm_col<__hip_bfloat16, __hip_bfloat16, float>(M, K, kernel, flags, B, C, sv, rv, vv, nullptr, 1.f, s);
+            case SWIFT_F16: return spmm_col<__half, __half, float>(M, K, kernel, flags, B, C, sv, rv, vv, kscale, gs, s);
         }
     }
     set_error("swift_spmm: unsupported dtype");

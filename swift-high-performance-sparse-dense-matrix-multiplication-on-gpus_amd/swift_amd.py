@@ -373,8 +373,9 @@ class SwiftMatrix:
         K = dC.shape[1]
         dB = torch.empty((self.n, K), dtype=dC.dtype, device=dC.device) if want_dB else None
         dv = torch.empty(self.nnz, dtype=dC.dtype, device=dC.device) if want_dvals else None
+        T = self.transpose()
         check("swift_backward_fused", lib().swift_backward_fused(
-            self.h, K, _dtype_code(dC.dtype), _ptr(dC.contiguous()),
+            T.h, K, _dtype_code(dC.dtype), _ptr(dC.contiguous()),
             _ptr(B.contiguous()) if B is not None else None, _ptr(dB), _ptr(dv), _stream()))
         return dB, dv
 

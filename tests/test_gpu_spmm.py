@@ -114,7 +114,8 @@ def test_b_sorted_flag(swift, oracle, cuda, layout, dt):
 
 @pytest.mark.parametrize("kernel", ["DEFAULT", "BALANCED"])
 @pytest.mark.parametrize("K", [32, 100])
-def test_accumulate_flag(swift, oracle, cuda, kernel, K):
+@pytest.mark.parametrize("layout", ["ROW", "COL"])
+def test_accumulate_flag(swift, oracle, cuda, kernel, K, layout):
     import torch
     m, n, rp, ci, v = CASES["powerlaw_3000x2500"]
     vv = np.asarray(v, np.float32)
@@ -122,10 +123,14 @@ def test_accumulate_flag(swift, oracle, cuda, kernel, K):
     C0 = np.random.default_rng(2).uniform(-1, 1, (m, K)).astype(np.float32)
     C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
     M = swift.SwiftMatrix(m, n, rp, ci, vv)
-    out = torch.tensor(C0, device="cuda")
-    M.spmm(torch.tensor(B, device="cuda"), out=out, flags=swift.ACCUMULATE,
-           kernel=getattr(swift, "KERNEL_" + kernel))
+    col = layout == "COL"
+    out = torch.tensor(C0.T.copy() if col else C0, device="cuda")
+    M.spmm(torch.tensor(B.T.copy() if col else B, device="cuda"), out=out,
+           flags=swift.ACCUMULATE, kernel=getattr(swift, "KERNEL_" + kernel),
+           layout=swift.COL_MAJOR if col else swift.ROW_MAJOR)
     got = out.cpu().numpy().astype(np.float64)
+    if col:
+        got = got.T
     assert gpu_common.rel_err(got - C0, C64, absC + np.abs(C0)) <= 1e-5
     M.close()
 
