@@ -166,6 +166,12 @@ void swift_destroy(swift_matrix *mat);
  * the sorted row keys (kept beside, not over, sortrowidx). */
 int swift_row_idx_sort(swift_matrix *mat, swift_stream_t stream);
 
+/* Schedule knob of the row-view kernels: items (rows + entries) per
+ * merge-path segment, i.e. per wave (default 128).  Rebuilds the segment
+ * table on the GPU; the format itself is unchanged.  The batch autotuner
+ * (swift_batch_create) sets it per matrix. */
+int swift_set_segment_items(swift_matrix *mat, int items, swift_stream_t stream);
+
 /* Sizes + device pointers of the format (valid until swift_destroy). */
 int swift_get_info(const swift_matrix *mat, swift_info *info);
 

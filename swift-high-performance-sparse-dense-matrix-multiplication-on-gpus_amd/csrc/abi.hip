@@ -14,6 +14,7 @@ namespace swift {
 int build_impl(swift_matrix *M, const int *d_rowptr, const int *d_colidx, const void *d_val,
                hipStream_t s);
 int row_sort_impl(swift_matrix *M, hipStream_t s);
+int build_segments(swift_matrix *M, int W, hipStream_t s);
 int permute_dense(const swift_matrix *M, int K, swift_layout layout, size_t es, const void *in,
                   void *out, hipStream_t s);
 void destroy_matrix(swift_matrix *M);
@@ -168,6 +169,14 @@ extern "C" int swift_row_idx_sort(swift_matrix *mat, swift_stream_t stream) {
         return SWIFT_ERR_ARG;
     }
     SWIFT_GUARD(return row_sort_impl(mat, S(stream));)
+}
+
+extern "C" int swift_set_segment_items(swift_matrix *mat, int items, swift_stream_t stream) {
+    if (!mat || items < 8 || items > (1 << 20)) {
+        set_error("swift_set_segment_items: items must be in [8, 2^20]");
+        return SWIFT_ERR_ARG;
+    }
+    SWIFT_GUARD(return build_segments(mat, items, S(stream));)
 }
 
 extern "C" int swift_get_info(const swift_matrix *M, swift_info *o) {

@@ -217,7 +217,7 @@ __device__ __forceinline__ T group_sum(T v) {
 template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
 __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     constexpr int EPS = 64 / LPE;
-    constexpr int UNROLL = 2;
+    constexpr int UNROLL = 4;
     constexpr int STEP = EPS * UNROLL;
     const int lane = threadIdx.x & 63;
     const int h = lane / LPE, sub = lane % LPE;
@@ -231,41 +231,61 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     const T *val = static_cast<const T *>(a.rv_val);
     const T *G = static_cast<const T *>(a.G);
     const T *H = static_cast<const T *>(a.H);
+    // staged row ends and entries, as in the forward row-view kernel
+    int rbase = row;
+    int rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+    int ebase = e, mc = 0, ms = 0;
+    T mv = (T)0;
+    if (ebase + lane < e1) {
+        mc = a.rv_col[ebase + lane];
+        ms = a.rv_src[ebase + lane];
+        mv = val[ebase + lane];
+    }
     T acc[Q][VEC], hr[Q][VEC];
 #pragma unroll
     for (int q = 0; q < Q; q++)
 #pragma unroll
-        for (int v = 0; v < VEC; v++) acc[q][v] = (T)0;
+        for (int v = 0; v < VEC; v++) acc[q][v] = hr[q][v] = (T)0;
     while (true) {
-        const int row_end = row < r1 ? a.rv_ptr[row + 1] : e1;
+        if (row - rbase >= 64) {
+            rbase = row;
+            rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+        }
+        const int rend = __shfl(rend_l, row - rbase, 64);
+        const int row_end = row < r1 ? rend : e1;
         if (DV && e < row_end) {
 #pragma unroll
             for (int q = 0; q < Q; q++) ldv<T, VEC>(H + (size_t)row * a.K + q * LPE * VEC + sub * VEC, hr[q]);
         }
         while (e < row_end) {
-            const int nb = min(64, row_end - e);
-            int mc = 0, ms = 0;
-            T mv = (T)0;
-            if (lane < nb) {
-                mc = a.rv_col[e + lane];
-                ms = a.rv_src[e + lane];
-                mv = val[e + lane];
+            if (e - ebase >= 64) {
+                ebase = e;
+                mc = ms = 0;
+                mv = (T)0;
+                if (ebase + lane < e1) {
+                    mc = a.rv_col[ebase + lane];
+                    ms = a.rv_src[ebase + lane];
+                    mv = val[ebase + lane];
+                }
             }
+            const int off = e - ebase;
+            const int nb = min(64 - off, row_end - e);
             for (int j = 0; j < nb; j += STEP) {
                 T g[UNROLL][Q][VEC];
                 T vv[UNROLL];
                 int src[UNROLL];
 #pragma unroll
                 for (int u = 0; u < UNROLL; u++) {
-                    const int slot = j + u * EPS + h;
-                    // unconditional shuffles: see the forward row-view kernel
-                    const int c = __shfl(mc, slot & 63, 64);
-                    src[u] = __shfl(ms, slot & 63, 64);
-                    const T vs = __shfl(mv, slot & 63, 64);
-                    vv[u] = slot < nb ? vs : (T)0;
+                    const int idx = j + u * EPS + h;
+                    // unconditional shuffles: a lane reading a masked-off source
+                    // lane would get a stale value
+                    const int c = __shfl(mc, (off + idx) & 63, 64);
+                    src[u] = __shfl(ms, (off + idx) & 63, 64);
+                    const T vs = __shfl(mv, (off + idx) & 63, 64);
+                    vv[u] = idx < nb ? vs : (T)0;
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
-                        if (slot < nb)
+                        if (idx < nb)
                             ldv<T, VEC>(G + (size_t)c * a.K + q * LPE * VEC + sub * VEC, g[u][q]);
                         else
 #pragma unroll
@@ -301,11 +321,11 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
         }
         if (last && !(r1 < a.m && e1 > a.rv_ptr[r1])) break;
 #pragma unroll
-        for (int off = LPE; off < 64; off <<= 1)
+        for (int o = LPE; o < 64; o <<= 1)
 #pragma unroll
             for (int q = 0; q < Q; q++)
 #pragma unroll
-                for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], off, 64);
+                for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
         if (h == 0) {
             T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K;
             const bool split = last || (row == st.x && carried_in);

@@ -478,8 +478,12 @@ __global__ void k_inv_perm(const int *__restrict__ perm, int n, int *__restrict_
 }
 
 // Merge path over rows and entries: the end of row i is item rv_ptr[i+1] + i,
-// entry e of row r is item e + r.  Segment s starts at item d = s*W; the rows
-// finished before d are those with rv_ptr[i+1] + i < d.
+// entry e of row r is item e + r.  Segment s nominally starts at item d = s*W
+// (the rows finished before d are those with rv_ptr[i+1] + i < d); a start
+// that falls inside row r is moved back to r's first entry when r began less
+// than W items earlier, so that only rows longer than a segment are split.
+// Starts stay non-decreasing: a moved start lies after the previous nominal
+// start.
 __global__ void k_mp_seg(int m, int R, int W, int nseg, const int *__restrict__ ptr,
                          int2 *__restrict__ seg) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -490,7 +494,10 @@ __global__ void k_mp_seg(int m, int R, int W, int nseg, const int *__restrict__ 
         int mid = (lo + hi) >> 1;
         if ((long long)ptr[mid + 1] + mid < d) lo = mid + 1; else hi = mid;
     }
-    seg[s] = make_int2(lo, (int)(d - lo));
+    int e = (int)(d - lo);
+    if (s > 0 && s < nseg && lo < m && e > ptr[lo] && (long long)ptr[lo] + lo > d - W)
+        e = ptr[lo];
+    seg[s] = make_int2(lo, e);
 }
 
 // A segment ending inside row r (entries of r before and after its end) makes

@@ -28,11 +28,14 @@ namespace swift {
 
 // ---------------------------------------------------------- type helpers --
 
-template <class T> struct Vec16;  // 16-byte vector of T
-template <> struct Vec16<float> { using type = float4; static constexpr int N = 4; };
-template <> struct Vec16<double> { using type = double2; static constexpr int N = 2; };
-template <> struct Vec16<__hip_bfloat16> { using type = uint4; static constexpr int N = 8; };
-template <> struct Vec16<__half> { using type = uint4; static constexpr int N = 8; };
+// Elements per lane per B load: 16-byte loads for FP32/FP64; 8-byte loads of
+// four 16-bit values, which keeps the lane shape (and so the per-entry
+// instruction count) of FP32 while moving half the bytes.
+template <class T> struct Vec16;
+template <> struct Vec16<float> { static constexpr int N = 4; };
+template <> struct Vec16<double> { static constexpr int N = 2; };
+template <> struct Vec16<__hip_bfloat16> { static constexpr int N = 4; };
+template <> struct Vec16<__half> { static constexpr int N = 4; };
 
 __device__ __forceinline__ float to_acc(float x) { return x; }
 __device__ __forceinline__ double to_acc(double x) { return x; }
@@ -54,6 +57,19 @@ __device__ __forceinline__ void load_b(const TB *__restrict__ p, TC (&o)[VEC]) {
     } else if constexpr (std::is_same<TB, double>::value) {
         double2 v = *reinterpret_cast<const double2 *>(p);
         o[0] = v.x; o[1] = v.y;
+    } else if constexpr (std::is_same<TB, __hip_bfloat16>::value && VEC == 4) {
+        uint2 v = *reinterpret_cast<const uint2 *>(p);
+        o[0] = __uint_as_float(v.x << 16);
+        o[1] = __uint_as_float(v.x & 0xFFFF0000u);
+        o[2] = __uint_as_float(v.y << 16);
+        o[3] = __uint_as_float(v.y & 0xFFFF0000u);
+    } else if constexpr (std::is_same<TB, __half>::value && VEC == 4) {
+        uint2 v = *reinterpret_cast<const uint2 *>(p);
+        __half2 h0 = *reinterpret_cast<__half2 *>(&v.x), h1 = *reinterpret_cast<__half2 *>(&v.y);
+        o[0] = __low2float(h0);
+        o[1] = __high2float(h0);
+        o[2] = __low2float(h1);
+        o[3] = __high2float(h1);
     } else if constexpr (std::is_same<TB, __hip_bfloat16>::value) {
         uint4 v = *reinterpret_cast<const uint4 *>(p);
         unsigned w[4] = {v.x, v.y, v.z, v.w};

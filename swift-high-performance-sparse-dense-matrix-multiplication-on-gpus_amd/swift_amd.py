@@ -51,7 +51,7 @@ EXPORTS = (
     "swift_last_error", "swift_version", "swift_mmio_allinone", "swift_mmio_allinone_f32",
     "swift_host_free", "swift_generate", "swift_generate_dense", "swift_csr_to_csc",
     "swift_col_sort", "swift_build", "swift_build_host", "swift_destroy",
-    "swift_row_idx_sort", "swift_get_info", "swift_permute_dense", "swift_spmm",
+    "swift_row_idx_sort", "swift_set_segment_items", "swift_get_info", "swift_permute_dense", "swift_spmm",
     "swift_spmm_timed",
     "swift_spmm_host", "swift_prepare_values", "swift_convert_dense", "swift_spmm_scaled",
     "swift_build_transpose", "swift_sddmm", "swift_backward_fused", "swift_update_values",
@@ -89,6 +89,7 @@ def lib():
         L.swift_destroy.restype = None
         L.swift_get_info.argtypes = [_vp, _P(Info)]
         L.swift_row_idx_sort.argtypes = [_vp, _vp]
+        L.swift_set_segment_items.argtypes = [_vp, C.c_int, _vp]
         L.swift_spmm.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, _vp, _vp,
                                  _vp]
         L.swift_spmm_timed.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, _vp,
@@ -292,6 +293,10 @@ class SwiftMatrix:
                     raise RuntimeError("hipMemcpy failed: %d" % rc)
             out[k] = arr
         return out
+
+    def set_segment_items(self, items):
+        check("swift_set_segment_items", lib().swift_set_segment_items(self.h, items, _stream()))
+        return self
 
     def row_idx_sort(self):
         check("swift_row_idx_sort", lib().swift_row_idx_sort(self.h, _stream()))
