@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--K", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--extras", action="store_true", help="also time the other forms")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="also time the batched form over this many generated matrices")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the Swift SpMM path runs on a single MI355X "
@@ -164,6 +166,9 @@ def main():
     M.close()
     del B, C
     torch.cuda.empty_cache()
+    if args.batch:
+        out["batched"] = batched(S, torch, args.batch, args.scale)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         t0 = time.time()
         out["cpu_baseline"] = cpu_baseline(d, K, seed)
@@ -212,6 +217,60 @@ def extras(S, torch, M, d, B, K):
     bw = time_call(torch, lambda: M.backward_fused(dC, B), reps=5)
     res["fwd_bwd_over_fwd"] = round((base + bw) / base, 3)
     res["f32_row_gflops"] = round(flops / base / 1e6, 1)
+    return res
+
+
+def batched(S, torch, count, scale=1.0, reps=10):
+    """Batched form: `count` generated matrices (square, rows log-U[1k,64k],
+    mean nonzeros per row log-U[4,128], Zipf(2.0) rows, K in {32,64,128},
+    FP32 N(0,1)), all products in one persistent launch (plain and replayed
+    from a HIP graph) against one swift_spmm call per matrix."""
+    import ctypes as C
+    L = S.lib()
+    t0 = time.time()
+    mats, Ks, Bs, Cs = [], [], [], []
+    flops = 0.0
+    nnz_total = 0
+    g = torch.Generator(device="cuda").manual_seed(0)
+    for i in range(count):
+        d = S.generate(5, scale=scale, seed=0, index=i)
+        M = S.SwiftMatrix(d["m"], d["n"], d["rowptr"], d["colidx"], d["val"].astype(np.float32),
+                          dtype=torch.float32)
+        K = d["k"]
+        mats.append(M)
+        Ks.append(K)
+        Bs.append(torch.rand((d["n"], K), device="cuda", generator=g) * 2 - 1)
+        Cs.append(torch.empty((d["m"], K), device="cuda"))
+        flops += 2.0 * d["nnz"] * K
+        nnz_total += d["nnz"]
+    torch.cuda.synchronize()
+    log("batched: %d matrices, %d nnz, built in %.1fs" % (count, nnz_total, time.time() - t0))
+    stream = S._stream()
+    handles = [m.h for m in mats]
+    bptr = [S._ptr(b) for b in Bs]
+    cptr = [S._ptr(c) for c in Cs]
+
+    def separate():
+        for h, K, b, c in zip(handles, Ks, bptr, cptr):
+            L.swift_spmm(h, K, S.ROW_MAJOR, S.F32, S.KERNEL_DEFAULT, 0, b, c, stream)
+
+    res = {"matrices": count, "nnz": nnz_total, "gflop": round(flops / 1e9, 3)}
+    res["separate_default_ms"] = round(time_call(torch, separate, reps=reps), 4)
+    t0 = time.time()
+    batch = S.Batch(mats, Ks)
+    torch.cuda.synchronize()
+    res["batch_create_s"] = round(time.time() - t0, 2)
+    res["params"] = batch.params()
+    res["separate_tuned_ms"] = round(time_call(torch, separate, reps=reps), 4)
+    res["batch_ms"] = round(time_call(torch, lambda: batch.spmm(Bs, Cs), reps=reps), 4)
+    res["graph_ms"] = round(time_call(torch, lambda: batch.spmm(Bs, Cs, graph=True), reps=reps), 4)
+    for k in ("separate_default", "separate_tuned", "batch", "graph"):
+        res[k + "_gflops"] = round(flops / res[k + "_ms"] / 1e6, 1)
+    res["graph_speedup_vs_separate"] = round(res["separate_default_ms"] / res["graph_ms"], 3)
+    log("batched: %s" % res)
+    batch.close()
+    for M in mats:
+        M.close()
     return res
 
 

@@ -258,9 +258,15 @@ int swift_update_values(swift_matrix *mat, const void *d_csr_val,
 
 typedef struct swift_batch swift_batch; /* opaque */
 
-/* Group `count` built matrices (F32) with their K for one persistent launch.
- * The autotuner picks the work-unit depth and block shape per batch from K,
- * density and the regular/irregular ratio. */
+/* Group `count` built matrices (F32, K in {32, 64, 128, 256}) for one
+ * persistent launch.  No counterpart in the reference, which runs one
+ * matrix per process (test/k-32/FP32/main.cu:32-72) with two launches per
+ * product (test/k-32/FP32/Swift_gpu_row_major.h:264,280); here one work list
+ * of (matrix, segment) items serves every matrix.  The autotuner keys matrices by
+ * (K, log2 mean nonzeros per row, reside share), times the largest of each
+ * key over segment sizes {64, 128, 256, 512} and applies the fastest to all
+ * of that key (cached per process).  Operand pointers and 16-byte alignment
+ * are checked at launch. */
 int swift_batch_create(swift_batch **out, swift_matrix *const *mats,
                        const int *Ks, int count, swift_stream_t stream);
 /* All C_i = A_i * B_i (row-major, original column order, F32) in one
@@ -270,10 +276,19 @@ int swift_batch_spmm(swift_batch *batch, const float *const *d_B,
 /* Capture swift_batch_spmm into a HIP graph once, then replay it. */
 int swift_batch_graph_launch(swift_batch *batch, const float *const *d_B,
                              float *const *d_C, swift_stream_t stream);
-/* Tuned parameters (units, depth, block threads) for reporting. */
+/* Tuned parameters for reporting: work items of the persistent kernel, the
+ * most common segment size, threads per block. */
 int swift_batch_params(const swift_batch *batch, int *work_units,
                        int *depth, int *block_threads);
 void swift_batch_destroy(swift_batch *batch);
+
+/* Time the row-view product of one matrix for segment sizes {64, 128, 256,
+ * 512} on the caller's operands (C is overwritten) and keep the fastest.
+ * Has no counterpart in the reference, whose tiling constants are fixed at
+ * compile time (test/k-32/FP32/main.cu:21-23). */
+int swift_autotune(swift_matrix *mat, int K, swift_dtype dtype,
+                   swift_layout layout, const void *d_B, void *d_C,
+                   swift_stream_t stream, int *best_items);
 
 #ifdef __cplusplus
 }

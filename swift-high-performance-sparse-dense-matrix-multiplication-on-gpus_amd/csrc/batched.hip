@@ -1,28 +1,453 @@
-// batched.hip -- many Swift SpMMs in one persistent launch (placeholder until
-// the persistent kernel lands; every entry point reports UNSUPPORTED).
-#include "swift_internal.h"
+// batched.hip -- many uneven Swift SpMMs in one persistent launch, and the
+// autotuner that picks each matrix's row-view schedule.
+//
+// A batch is a list of built matrices (FP32, row-major, K in {32, 64, 128,
+// 256} per matrix).  Their row-view segments are flattened into one work list
+// of (matrix, segment) items; a persistent kernel sized to the device (32
+// waves per CU) hands items to waves through an atomic counter, and each wave
+// runs the same walk as the single-matrix row-view kernel with the matrix's
+// own lane shape (chosen by a wave-uniform switch on K).  Split rows are
+// zeroed first and the reside blocks of all matrices follow in one more
+// kernel.  swift_batch_graph_launch captures the four launches (counter reset,
+// split-row zeroing, persistent walk, reside blocks) into a HIP graph once and
+// replays it; new B/C pointers only rewrite the descriptor table.
+//
+// Autotuner: matrices are keyed by (K, log2 mean nonzeros per row, reside
+// share bucket); for each key the largest matrix is timed with segment sizes
+// {64, 128, 256, 512} and the fastest is applied to every matrix of that key.
+// Results are cached per process.
+#include "rowview.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+namespace swift {
+
+int build_segments(swift_matrix *M, int W, hipStream_t s);
+int spmm_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dtype,
+              swift_kernel kernel, unsigned flags, const void *B, void *C, const float *kscale,
+              hipStream_t s);
+
+// ------------------------------------------------------------ kernels --
+
+__global__ void k_batch_reset(int *counter) { *counter = 0; }
+
+// (matrix, row) pairs: zero the row of that matrix's C.
+__global__ __launch_bounds__(256) void k_batch_zero(const RowArgs *__restrict__ desc,
+                                                    const int2 *__restrict__ rows, int n) {
+    const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (i >= n) return;
+    const int2 r = rows[i];
+    const int K = desc[r.x].ldc;
+    float *c = static_cast<float *>(desc[r.x].C) + (size_t)r.y * K;
+    for (int k = threadIdx.x & 63; k < K; k += 64) c[k] = 0.0f;
+}
+
+template <int LPE>
+__device__ __forceinline__ void batch_walk(const RowArgs &a, int sid) {
+    rowview_walk<float, float, float, 4, 1, LPE, false>(a, sid);
+}
+
+// Persistent: every wave takes items until the counter passes nitems (an exit
+// every wave reaches).
+__global__ __launch_bounds__(256) void k_batch_walk(const RowArgs *__restrict__ desc,
+                                                    const int2 *__restrict__ items, int nitems,
+                                                    int *__restrict__ counter) {
+    for (;;) {
+        int it = 0;
+        if ((threadIdx.x & 63) == 0) it = atomicAdd(counter, 1);
+        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
+        if (it >= nitems) break;
+        const int2 w = items[it];
+        const int mi = __builtin_amdgcn_readfirstlane(w.x);
+        const int sid = __builtin_amdgcn_readfirstlane(w.y);
+        const RowArgs a = desc[mi];
+        switch (a.ldb) {
+            case 32: batch_walk<8>(a, sid); break;
+            case 64: batch_walk<16>(a, sid); break;
+            case 128: batch_walk<32>(a, sid); break;
+            default: batch_walk<64>(a, sid); break;  // 256
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_reside(const RowArgs *__restrict__ desc,
+                                                      const int2 *__restrict__ items, int n) {
+    const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (i >= n) return;
+    const int2 w = items[i];
+    const int mi = __builtin_amdgcn_readfirstlane(w.x);
+    const int blk = __builtin_amdgcn_readfirstlane(w.y);
+    const RowArgs a = desc[mi];
+    switch (a.ldb) {
+        case 32: reside_block<float, float, float, 4, 1, 8, false>(a, blk); break;
+        case 64: reside_block<float, float, float, 4, 1, 16, false>(a, blk); break;
+        case 128: reside_block<float, float, float, 4, 1, 32, false>(a, blk); break;
+        default: reside_block<float, float, float, 4, 1, 64, false>(a, blk); break;
+    }
+}
+
+// ---------------------------------------------------------- autotuner --
+
+using TuneKey = std::tuple<int, int, int>;
+static std::map<TuneKey, int> g_tuned;
+static std::mutex g_tune_mu;
+
+static TuneKey tune_key(const swift_matrix *M, int K) {
+    const double mean = M->m ? (double)M->nnz / M->m : 0.0;
+    const int dens = (int)std::floor(std::log2(mean + 1.0));
+    const double share = M->nnz ? (double)M->reside_nnz / M->nnz : 0.0;
+    const int res = share == 0.0 ? 0 : share < 0.01 ? 1 : share < 0.1 ? 2 : 3;
+    return TuneKey(K, dens, res);
+}
+
+// Times the row-view product of M for each candidate segment size on the
+// given operands and keeps the fastest (median of 3 after one warm-up).
+int autotune_impl(swift_matrix *M, int K, swift_dtype dtype, swift_layout layout, const void *B,
+                  void *C, hipStream_t s, int *best_out) {
+    static const int cands[] = {64, 128, 256, 512};
+    hipEvent_t e0, e1;
+    SWIFT_HIP(hipEventCreate(&e0));
+    SWIFT_HIP(hipEventCreate(&e1));
+    int best = M->mp_items ? M->mp_items : 128;
+    float best_ms = 1e30f;
+    int rc = SWIFT_OK;
+    for (int W : cands) {
+        if ((rc = build_segments(M, W, s)) != SWIFT_OK) break;
+        std::vector<float> t;
+        for (int r = 0; r < 4 && rc == SWIFT_OK; r++) {
+            (void)hipEventRecord(e0, s);
+            rc = spmm_impl(M, K, layout, dtype, SWIFT_KERNEL_DEFAULT, 0, B, C, nullptr, s);
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (r) t.push_back(ms);
+        }
+        if (rc != SWIFT_OK) break;
+        std::sort(t.begin(), t.end());
+        if (t[1] < best_ms) {
+            best_ms = t[1];
+            best = W;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != SWIFT_OK) return rc;
+    SWIFT_TRY(build_segments(M, best, s));
+    if (best_out) *best_out = best;
+    return SWIFT_OK;
+}
+
+}  // namespace swift
+
+using namespace swift;
 
 struct swift_batch {
-    int count = 0;
+    std::vector<swift_matrix *> mats;
+    std::vector<int> Ks, W;
+    std::vector<RowArgs> desc_h;
+    RowArgs *desc = nullptr;
+    int2 *items = nullptr, *res_items = nullptr, *split = nullptr;
+    int nitems = 0, nres = 0, nsplit = 0;
+    int *counter = nullptr;
+    int grid = 0;
+    std::vector<const float *> lastB;
+    std::vector<float *> lastC;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
 };
 
-extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *, const int *, int,
-                                  swift_stream_t) {
-    if (out) *out = nullptr;
-    swift::set_error("swift_batch_create: not built yet");
-    return SWIFT_ERR_UNSUPPORTED;
+static void batch_free(swift_batch *b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void *p : {(void *)b->desc, (void *)b->items, (void *)b->res_items, (void *)b->split,
+                    (void *)b->counter})
+        if (p) (void)hipFree(p);
+    delete b;
 }
-extern "C" int swift_batch_spmm(swift_batch *, const float *const *, float *const *,
-                                swift_stream_t) {
-    swift::set_error("swift_batch_spmm: not built yet");
-    return SWIFT_ERR_UNSUPPORTED;
+
+static RowArgs row_args_of(const swift_matrix *M, int K) {
+    RowArgs a{};
+    a.nblk = M->numblk;
+    a.colorblk = M->colorblk;
+    a.blkstart = M->blkstart;
+    a.blkstop = M->blkstop;
+    a.rptr = M->reside_cscptr;
+    a.rrow = M->reside_cscrowidx;
+    a.rval = M->reside_val;
+    a.m = M->m;
+    a.nseg = M->nseg;
+    a.seg = M->seg;
+    a.rv_ptr = M->rv_ptr;
+    a.rv_col = M->rv_col;
+    a.rv_val = M->rv_val;
+    a.n = M->n;
+    a.perm = M->perm;
+    a.ldb = K;
+    a.ldc = K;
+    a.kbase = 0;
+    a.kcount = K;
+    a.gscale = 1.0f;
+    return a;
 }
-extern "C" int swift_batch_graph_launch(swift_batch *, const float *const *, float *const *,
-                                        swift_stream_t) {
-    swift::set_error("swift_batch_graph_launch: not built yet");
-    return SWIFT_ERR_UNSUPPORTED;
+
+extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, const int *Ks,
+                                  int count, swift_stream_t stream) {
+    if (!out || !mats || !Ks || count <= 0) {
+        set_error("swift_batch_create: bad arguments");
+        return SWIFT_ERR_ARG;
+    }
+    *out = nullptr;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    for (int i = 0; i < count; i++) {
+        const int K = Ks[i];
+        if (!mats[i] || mats[i]->vtype != SWIFT_F32 || !(K == 32 || K == 64 || K == 128 || K == 256)) {
+            set_error("swift_batch_create: matrix %d must be FP32 with K in {32,64,128,256}", i);
+            return SWIFT_ERR_UNSUPPORTED;
+        }
+    }
+    swift_batch *b = new (std::nothrow) swift_batch();
+    if (!b) return SWIFT_ERR_ALLOC;
+    int rc = SWIFT_OK;
+    try {
+        b->mats.assign(mats, mats + count);
+        b->Ks.assign(Ks, Ks + count);
+        b->W.assign(count, 128);
+        // autotune one representative per key
+        std::map<TuneKey, int> rep;
+        for (int i = 0; i < count; i++) {
+            TuneKey k = tune_key(mats[i], Ks[i]);
+            auto it = rep.find(k);
+            if (it == rep.end() || mats[i]->nnz > mats[it->second]->nnz) rep[k] = i;
+        }
+        for (auto &kv : rep) {
+            int W = 0;
+            {
+                std::lock_guard<std::mutex> g(g_tune_mu);
+                auto it = g_tuned.find(kv.first);
+                if (it != g_tuned.end()) W = it->second;
+            }
+            if (!W) {
+                swift_matrix *M = mats[kv.second];
+                const int K = Ks[kv.second];
+                Buffer Bt, Ct;
+                if ((rc = Bt.alloc(sizeof(float) * (size_t)M->n * K + 16)) != SWIFT_OK) break;
+                if ((rc = Ct.alloc(sizeof(float) * (size_t)M->m * K + 16)) != SWIFT_OK) break;
+                if ((rc = fill_bytes_zero(Bt.ptr, sizeof(float) * (size_t)M->n * K, s)) != SWIFT_OK) break;
+                if ((rc = autotune_impl(M, K, SWIFT_F32, SWIFT_ROW_MAJOR, Bt.ptr, Ct.ptr, s, &W)) != SWIFT_OK) break;
+                (void)hipStreamSynchronize(s);
+                std::lock_guard<std::mutex> g(g_tune_mu);
+                g_tuned[kv.first] = W;
+            }
+            for (int i = 0; i < count; i++)
+                if (tune_key(mats[i], Ks[i]) == kv.first) b->W[i] = W;
+        }
+        if (rc == SWIFT_OK) {
+            // a matrix listed more than once keeps one segment table
+            std::map<const swift_matrix *, int> first;
+            for (int i = 0; i < count; i++) {
+                auto it = first.emplace(mats[i], b->W[i]).first;
+                b->W[i] = it->second;
+            }
+            for (int i = 0; i < count && rc == SWIFT_OK; i++)
+                if (mats[i]->mp_items != b->W[i]) rc = build_segments(mats[i], b->W[i], s);
+        }
+        if (rc == SWIFT_OK) {
+            // flattened work lists (host) -> device
+            std::vector<int2> items, res, split;
+            std::vector<int> rows;
+            for (int i = 0; i < count; i++) {
+                swift_matrix *M = mats[i];
+                for (int q = 0; q < M->nseg; q++) items.push_back(make_int2(i, q));
+                for (int q = 0; q < M->numblk; q++) res.push_back(make_int2(i, q));
+                rows.resize((size_t)M->nsplit);
+                if (M->nsplit &&
+                    hipMemcpy(rows.data(), M->split_rows, sizeof(int) * M->nsplit,
+                              hipMemcpyDeviceToHost) != hipSuccess) {
+                    set_error("swift_batch_create: copying split rows failed");
+                    rc = SWIFT_ERR_HIP;
+                    break;
+                }
+                for (int r : rows) split.push_back(make_int2(i, r));
+                b->desc_h.push_back(row_args_of(M, b->Ks[i]));
+            }
+            auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+                if (hipMalloc(dst, std::max<size_t>(bytes, 16)) != hipSuccess) {
+                    set_error("swift_batch_create: hipMalloc(%zu) failed", bytes);
+                    return SWIFT_ERR_ALLOC;
+                }
+                if (bytes && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+                    set_error("swift_batch_create: upload failed");
+                    return SWIFT_ERR_HIP;
+                }
+                return SWIFT_OK;
+            };
+            if (rc == SWIFT_OK) rc = up((void **)&b->items, items.data(), items.size() * sizeof(int2));
+            if (rc == SWIFT_OK) rc = up((void **)&b->res_items, res.data(), res.size() * sizeof(int2));
+            if (rc == SWIFT_OK) rc = up((void **)&b->split, split.data(), split.size() * sizeof(int2));
+            if (rc == SWIFT_OK)
+                rc = up((void **)&b->desc, b->desc_h.data(), b->desc_h.size() * sizeof(RowArgs));
+            if (rc == SWIFT_OK) rc = up((void **)&b->counter, nullptr, 0);
+            b->nitems = (int)items.size();
+            b->nres = (int)res.size();
+            b->nsplit = (int)split.size();
+            int dev = 0, cus = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            b->grid = std::max(1, cus) * 8;  // 8 blocks x 4 waves = 32 waves per CU
+            b->lastB.assign(count, nullptr);
+            b->lastC.assign(count, nullptr);
+        }
+    } catch (...) {
+        set_error("swift_batch_create: host allocation failed");
+        rc = SWIFT_ERR_ALLOC;
+    }
+    if (rc != SWIFT_OK) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return SWIFT_OK;
 }
-extern "C" int swift_batch_params(const swift_batch *, int *, int *, int *) {
-    return SWIFT_ERR_UNSUPPORTED;
+
+static int batch_set_operands(swift_batch *b, const float *const *dB, float *const *dC,
+                              hipStream_t s, bool *changed) {
+    if (!dB || !dC) {
+        set_error("swift_batch_spmm: null operand arrays");
+        return SWIFT_ERR_ARG;
+    }
+    bool ch = false;
+    for (size_t i = 0; i < b->mats.size(); i++) {
+        const swift_matrix *M = b->mats[i];
+        if (M->seg != b->desc_h[i].seg || M->nseg != b->desc_h[i].nseg ||
+            M->mp_items != b->W[i]) {
+            set_error("swift_batch_spmm: matrix %zu was re-segmented after swift_batch_create", i);
+            return SWIFT_ERR_ARG;
+        }
+        if (!dB[i] || !dC[i]) {
+            set_error("swift_batch_spmm: null operand %zu", i);
+            return SWIFT_ERR_ARG;
+        }
+        if (((uintptr_t)dB[i] | (uintptr_t)dC[i]) % 16) {
+            set_error("swift_batch_spmm: operands of matrix %zu are not 16-byte aligned", i);
+            return SWIFT_ERR_ARG;
+        }
+        if (dB[i] != b->lastB[i] || dC[i] != b->lastC[i]) {
+            b->desc_h[i].B = dB[i];
+            b->desc_h[i].C = dC[i];
+            b->lastB[i] = dB[i];
+            b->lastC[i] = dC[i];
+            ch = true;
+        }
+    }
+    if (ch) {
+        SWIFT_HIP(hipMemcpyAsync(b->desc, b->desc_h.data(), b->desc_h.size() * sizeof(RowArgs),
+                                 hipMemcpyHostToDevice, s));
+        SWIFT_HIP(hipStreamSynchronize(s));  // desc_h may change on the next call
+    }
+    if (changed) *changed = ch;
+    return SWIFT_OK;
 }
-extern "C" void swift_batch_destroy(swift_batch *b) { delete b; }
+
+static int batch_launch(swift_batch *b, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, b->counter);
+    if (b->nsplit)
+        hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div((size_t)b->nsplit * 64, 256)), dim3(256), 0,
+                           s, b->desc, b->split, b->nsplit);
+    if (b->nitems)
+        hipLaunchKernelGGL(k_batch_walk, dim3(b->grid), dim3(256), 0, s, b->desc, b->items,
+                           b->nitems, b->counter);
+    if (b->nres)
+        hipLaunchKernelGGL(k_batch_reside, dim3(ceil_div((size_t)b->nres * 64, 256)), dim3(256), 0,
+                           s, b->desc, b->res_items, b->nres);
+    SWIFT_CHECK_LAUNCH();
+    return SWIFT_OK;
+}
+
+extern "C" int swift_batch_spmm(swift_batch *b, const float *const *d_B, float *const *d_C,
+                                swift_stream_t stream) {
+    if (!b) {
+        set_error("swift_batch_spmm: null batch");
+        return SWIFT_ERR_ARG;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    SWIFT_TRY(batch_set_operands(b, d_B, d_C, s, nullptr));
+    return batch_launch(b, s);
+}
+
+extern "C" int swift_batch_graph_launch(swift_batch *b, const float *const *d_B,
+                                        float *const *d_C, swift_stream_t stream) {
+    if (!b) {
+        set_error("swift_batch_graph_launch: null batch");
+        return SWIFT_ERR_ARG;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    SWIFT_TRY(batch_set_operands(b, d_B, d_C, s, nullptr));
+    if (!b->exec) {
+        // capture on a private stream (capture is not allowed on the legacy
+        // null stream), then replay on the caller's stream
+        hipStream_t cs;
+        SWIFT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        int rc = e == hipSuccess ? batch_launch(b, cs) : SWIFT_ERR_HIP;
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cs, &g);
+        (void)hipStreamDestroy(cs);
+        if (e != hipSuccess || e2 != hipSuccess || rc != SWIFT_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            set_error("swift_batch_graph_launch: capture failed (%s / %s)", hipGetErrorString(e),
+                      hipGetErrorString(e2));
+            return rc != SWIFT_OK ? rc : SWIFT_ERR_HIP;
+        }
+        b->graph = g;
+        SWIFT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    }
+    SWIFT_HIP(hipGraphLaunch(b->exec, s));
+    return SWIFT_OK;
+}
+
+extern "C" int swift_batch_params(const swift_batch *b, int *work_units, int *depth,
+                                  int *block_threads) {
+    if (!b) {
+        set_error("swift_batch_params: null batch");
+        return SWIFT_ERR_ARG;
+    }
+    if (work_units) *work_units = b->nitems;
+    if (depth) {  // most common tuned segment size
+        std::map<int, int> cnt;
+        for (int w : b->W) cnt[w]++;
+        int best = 0, bc = -1;
+        for (auto &kv : cnt)
+            if (kv.second > bc) { bc = kv.second; best = kv.first; }
+        *depth = best;
+    }
+    if (block_threads) *block_threads = 256;
+    return SWIFT_OK;
+}
+
+extern "C" void swift_batch_destroy(swift_batch *b) {
+    if (b) (void)hipDeviceSynchronize();
+    batch_free(b);
+}
+
+extern "C" int swift_autotune(swift_matrix *mat, int K, swift_dtype dtype, swift_layout layout,
+                              const void *d_B, void *d_C, swift_stream_t stream, int *best_items) {
+    if (!mat || K <= 0 || !d_B || !d_C) {
+        set_error("swift_autotune: bad arguments");
+        return SWIFT_ERR_ARG;
+    }
+    try {
+        return autotune_impl(mat, K, dtype, layout, d_B, d_C, reinterpret_cast<hipStream_t>(stream),
+                             best_items);
+    } catch (...) {
+        set_error("swift_autotune: host allocation failed");
+        return SWIFT_ERR_ALLOC;
+    }
+}

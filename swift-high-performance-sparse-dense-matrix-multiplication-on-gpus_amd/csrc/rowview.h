@@ -1,0 +1,362 @@
+// rowview.h -- device building blocks of the row-major Swift kernels (gfx950),
+// shared by the single-matrix launches (spmm.hip) and the batched persistent
+// kernel (batched.hip).
+#pragma once
+
+#include "swift_internal.h"
+
+#include <type_traits>
+
+namespace swift {
+
+// ---------------------------------------------------------- type helpers --
+
+// Elements per lane per B load: 16-byte loads for FP32/FP64; 8-byte loads of
+// four 16-bit values, which keeps the lane shape (and so the per-entry
+// instruction count) of FP32 while moving half the bytes.
+template <class T> struct Vec16;
+template <> struct Vec16<float> { static constexpr int N = 4; };
+template <> struct Vec16<double> { static constexpr int N = 2; };
+template <> struct Vec16<__hip_bfloat16> { static constexpr int N = 4; };
+template <> struct Vec16<__half> { static constexpr int N = 4; };
+
+__device__ __forceinline__ float to_acc(float x) { return x; }
+__device__ __forceinline__ double to_acc(double x) { return x; }
+__device__ __forceinline__ float to_acc(__hip_bfloat16 x) { return __bfloat162float(x); }
+__device__ __forceinline__ float to_acc(__half x) { return __half2float(x); }
+
+template <class TA> struct AccOf { using type = float; };
+template <> struct AccOf<double> { using type = double; };
+
+// Load VEC consecutive elements of B starting at p (16-byte aligned when
+// VEC > 1) and widen them to the accumulator type.
+template <class TB, int VEC, class TC>
+__device__ __forceinline__ void load_b(const TB *__restrict__ p, TC (&o)[VEC]) {
+    if constexpr (VEC == 1) {
+        o[0] = to_acc(p[0]);
+    } else if constexpr (std::is_same<TB, float>::value) {
+        float4 v = *reinterpret_cast<const float4 *>(p);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    } else if constexpr (std::is_same<TB, double>::value) {
+        double2 v = *reinterpret_cast<const double2 *>(p);
+        o[0] = v.x; o[1] = v.y;
+    } else if constexpr (std::is_same<TB, __hip_bfloat16>::value && VEC == 4) {
+        uint2 v = *reinterpret_cast<const uint2 *>(p);
+        o[0] = __uint_as_float(v.x << 16);
+        o[1] = __uint_as_float(v.x & 0xFFFF0000u);
+        o[2] = __uint_as_float(v.y << 16);
+        o[3] = __uint_as_float(v.y & 0xFFFF0000u);
+    } else if constexpr (std::is_same<TB, __half>::value && VEC == 4) {
+        uint2 v = *reinterpret_cast<const uint2 *>(p);
+        __half2 h0 = *reinterpret_cast<__half2 *>(&v.x), h1 = *reinterpret_cast<__half2 *>(&v.y);
+        o[0] = __low2float(h0);
+        o[1] = __high2float(h0);
+        o[2] = __low2float(h1);
+        o[3] = __high2float(h1);
+    } else if constexpr (std::is_same<TB, __hip_bfloat16>::value) {
+        uint4 v = *reinterpret_cast<const uint4 *>(p);
+        unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            o[2 * i] = __uint_as_float(w[i] << 16);
+            o[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+        }
+    } else {  // __half
+        uint4 v = *reinterpret_cast<const uint4 *>(p);
+        unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            __half2 h = *reinterpret_cast<__half2 *>(&w[i]);
+            o[2 * i] = __low2float(h);
+            o[2 * i + 1] = __high2float(h);
+        }
+    }
+}
+
+__device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------- row-major kernels --
+//
+// Lanes run along K in groups of LPE lanes (one entry stream per group, EPS =
+// 64/LPE groups per wave), VEC elements per lane per chunk, Q chunks.  Two
+// element layouts are used:
+//   strided     k = q*LPE*VEC + v*LPE + sub   (atomics: for a fixed (q, v) the
+//               LPE lanes hit LPE consecutive elements, i.e. whole lines)
+//   contiguous  k = q*LPE*VEC + sub*VEC + v   (16-byte loads and stores)
+
+struct RowArgs {
+    // tile-order kernels
+    int nwork;
+    const int2 *work;           // (first tile, tile count) per work item
+    const int *tile_colidx;
+    const int *srow;            // sortrowidx
+    const void *sval;           // sortval (TV)
+    // reside (irregular) kernels
+    int nblk;                   // balance blocks (or reside columns)
+    const unsigned *colorblk;   // nullptr => unbalanced, one block per column
+    const int *blkstart, *blkstop;
+    const int *rptr, *rrow;
+    const void *rval;
+    // row-view kernel
+    int m, nseg;
+    const int2 *seg;
+    const int *rv_ptr, *rv_col;
+    const void *rv_val;
+    const int *inv_perm;        // B in sorted order: B row of column c is inv_perm[c]
+    int accumulate;
+    // shared
+    int n;                      // columns of A
+    const int *perm;            // tile-order: nullptr when B is already sorted
+    const void *B;
+    void *C;
+    int ldb, ldc;               // row strides (elements)
+    int kbase, kcount;          // K slice handled by this launch
+    const float *kscale;        // F16: per-k output scale (nullptr otherwise)
+    float gscale;               // F16: A-value scale
+};
+
+template <class TC>
+__device__ __forceinline__ TC out_scale(const RowArgs &a, int k) {
+    return a.kscale ? (TC)(a.kscale[a.kbase + k] * a.gscale) : (TC)a.gscale;
+}
+
+template <class TC, int VEC, int Q, int LPE, bool MASK>
+__device__ __forceinline__ void flush_strided(const RowArgs &a, int row, int sub,
+                                              TC (&acc)[Q][VEC]) {
+    TC *crow = static_cast<TC *>(a.C) + (size_t)row * a.ldc + a.kbase;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) {
+            const int k = q * LPE * VEC + v * LPE + sub;
+            if (!MASK || k < a.kcount) atomic_add(crow + k, acc[q][v] * out_scale<TC>(a, k));
+            acc[q][v] = (TC)0;
+        }
+}
+
+template <class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+__device__ __forceinline__ void load_strided(const RowArgs &a, int bcol, int sub,
+                                             TC (&b)[Q][VEC]) {
+    const TB *brow = static_cast<const TB *>(a.B) + (size_t)bcol * a.ldb + a.kbase;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) {
+            const int k = q * LPE * VEC + v * LPE + sub;
+            b[q][v] = (!MASK || k < a.kcount) ? to_acc(brow[k]) : (TC)0;
+        }
+}
+
+// -------------------------------------------------------- row view kernel --
+
+template <class TC, int VEC>
+__device__ __forceinline__ void store_vec(TC *p, const TC (&x)[VEC]) {
+    if constexpr (std::is_same<TC, float>::value && VEC % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < VEC; i += 4)
+            *reinterpret_cast<float4 *>(p + i) = make_float4(x[i], x[i + 1], x[i + 2], x[i + 3]);
+    } else if constexpr (std::is_same<TC, double>::value && VEC % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < VEC; i += 2)
+            *reinterpret_cast<double2 *>(p + i) = make_double2(x[i], x[i + 1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < VEC; i++) p[i] = x[i];
+    }
+}
+
+// Writes one finished row: plain (or read-modify-write when accumulating)
+// vector stores, or atomics for a row split across segments.
+template <class TC, int VEC, int Q, int LPE, bool MASK>
+__device__ __forceinline__ void put_row(const RowArgs &a, int row, int sub, bool split,
+                                        TC (&acc)[Q][VEC]) {
+    TC *crow = static_cast<TC *>(a.C) + (size_t)row * a.ldc + a.kbase;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int k0 = q * LPE * VEC + sub * VEC;
+        TC x[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; v++) x[v] = acc[q][v] * out_scale<TC>(a, k0 + v);
+        if (MASK) {
+#pragma unroll
+            for (int v = 0; v < VEC; v++)
+                if (k0 + v < a.kcount) {
+                    if (split) atomic_add(crow + k0 + v, x[v]);
+                    else crow[k0 + v] = a.accumulate ? crow[k0 + v] + x[v] : x[v];
+                }
+        } else if (split) {
+#pragma unroll
+            for (int v = 0; v < VEC; v++) atomic_add(crow + k0 + v, x[v]);
+        } else {
+            if (a.accumulate) {
+                TC old[VEC];
+                load_b<TC, VEC, TC>(crow + k0, old);
+#pragma unroll
+                for (int v = 0; v < VEC; v++) x[v] += old[v];
+            }
+            store_vec<TC, VEC>(crow + k0, x);
+        }
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+    }
+}
+
+// Block order -> logical block such that each XCD (blocks are dealt to the 8
+// XCDs round-robin) gets one contiguous range of logical blocks: neighbouring
+// segments cover neighbouring rows, whose B rows then meet in one L2.  Any
+// other dispatch order only costs locality, never correctness.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks) {
+    constexpr unsigned X = 8;
+    const unsigned per = nblocks / X, rem = nblocks % X, x = b % X;
+    return x * per + min(x, rem) + b / X;
+}
+
+// Merge-path walk of the row view: each wave owns one segment of W items
+// (rows + entries), so every wave does the same amount of work whatever the
+// row lengths.  The segment's entries are contiguous in the row view, so they
+// are staged 64 at a time (column, value) together with 64 row ends, and rows
+// are walked inside the staged batch: the only dependent loads are the B row
+// gathers.  Lanes split into EPS sub-groups of LPE lanes; sub-group h takes
+// entries h, h+EPS, ... with UNROLL 16-byte loads per lane in flight; at a
+// row end the sub-groups' sums are combined with a butterfly and sub-group 0
+// stores the row.  Control flow is uniform across the wave.  Rows cut by a
+// segment boundary are split rows (zeroed beforehand) and take atomics.
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+__device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
+    constexpr int EPS = 64 / LPE;
+    constexpr int UNROLL = 4;
+    constexpr int STEP = EPS * UNROLL;  // entries per iteration
+    const int lane = threadIdx.x & 63;
+    const int h = lane / LPE, sub = lane % LPE;
+    const int2 st = a.seg[sid], en = a.seg[sid + 1];
+    const int r1 = en.x, e1 = en.y;
+    int row = st.x, e = st.y;
+    const bool carried_in = row < a.m && e > a.rv_ptr[row];
+    const TV *val = static_cast<const TV *>(a.rv_val);
+    const TB *B = static_cast<const TB *>(a.B);
+    // staged row ends: lane l holds the end of row rbase + l
+    int rbase = row;
+    int rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+    // staged entries: lane l holds entry ebase + l
+    int ebase = e, mc = 0;
+    TC mv = (TC)0;
+    if (ebase + lane < e1) {
+        const int c = a.rv_col[ebase + lane];
+        mc = a.inv_perm ? a.inv_perm[c] : c;
+        mv = to_acc(val[ebase + lane]);
+    }
+    TC acc[Q][VEC];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+    while (true) {
+        if (row - rbase >= 64) {
+            rbase = row;
+            rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+        }
+        const int rend = __shfl(rend_l, row - rbase, 64);
+        const int row_end = row < r1 ? rend : e1;
+        while (e < row_end) {
+            if (e - ebase >= 64) {
+                ebase = e;
+                mc = 0;
+                mv = (TC)0;
+                if (ebase + lane < e1) {
+                    const int c = a.rv_col[ebase + lane];
+                    mc = a.inv_perm ? a.inv_perm[c] : c;
+                    mv = to_acc(val[ebase + lane]);
+                }
+            }
+            const int off = e - ebase;
+            const int nb = min(64 - off, row_end - e);  // entries of this row in the batch
+            for (int j = 0; j < nb; j += STEP) {
+                TC b[UNROLL][Q][VEC];
+                TC vv[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++) {
+                    const int idx = j + u * EPS + h;
+                    // every lane takes part in both shuffles (a lane reading a
+                    // source lane that is masked off would get a stale value)
+                    const int c = __shfl(mc, (off + idx) & 63, 64);
+                    const TC vs = __shfl(mv, (off + idx) & 63, 64);
+                    vv[u] = idx < nb ? vs : (TC)0;
+                    const TB *brow = B + (size_t)c * a.ldb + a.kbase;
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        const int k0 = q * LPE * VEC + sub * VEC;
+                        if (idx < nb && (!MASK || k0 < a.kcount))
+                            load_b<TB, VEC, TC>(brow + k0, b[u][q]);
+                        else
+#pragma unroll
+                            for (int v = 0; v < VEC; v++) b[u][q][v] = (TC)0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++)
+#pragma unroll
+                    for (int q = 0; q < Q; q++)
+#pragma unroll
+                        for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * b[u][q][v];
+            }
+            e += nb;
+        }
+        const bool last = row >= r1;
+        if (last && !(r1 < a.m && e1 > a.rv_ptr[r1])) break;
+        // combine the EPS sub-groups' partial sums of this row
+#pragma unroll
+        for (int o = LPE; o < 64; o <<= 1)
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
+        if (h == 0)
+            put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, last || (row == st.x && carried_in), acc);
+        else
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+        if (last) break;
+        row++;
+    }
+}
+
+// One balance block (or one reside column when unbalanced) per wave: the
+// block's single B row is loaded once, each entry is one row flush.
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+__device__ __forceinline__ void reside_block(const RowArgs &a, int blk) {
+    constexpr int EPS = 64 / LPE;
+    const int lane = threadIdx.x & 63;
+    const int h = lane / LPE, sub = lane % LPE;
+    int r, start, stop;
+    if (a.colorblk) {
+        const unsigned u = a.colorblk[blk];
+        r = (int)(u & 0x7FFFFFFFu);
+        const bool split = (u >> 31) != 0;
+        start = split ? a.blkstart[blk] : a.rptr[r];
+        stop = split ? a.blkstop[blk] : a.rptr[r + 1];
+    } else {
+        r = blk;
+        start = a.rptr[r];
+        stop = a.rptr[r + 1];
+    }
+    const int jj = a.n - 1 - r;
+    const int bcol = a.perm ? a.perm[jj] : jj;
+    TC b[Q][VEC];
+    load_strided<TB, TC, VEC, Q, LPE, MASK>(a, bcol, sub, b);
+    const TV *rval = static_cast<const TV *>(a.rval);
+    for (int e = start + h; e < stop; e += EPS) {
+        const int row = a.rrow[e];
+        const TC val = to_acc(rval[e]);
+        TC acc[Q][VEC];
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+            for (int v = 0; v < VEC; v++) acc[q][v] = val * b[q][v];
+        flush_strided<TC, VEC, Q, LPE, MASK>(a, row, sub, acc);
+    }
+}
+
+}  // namespace swift

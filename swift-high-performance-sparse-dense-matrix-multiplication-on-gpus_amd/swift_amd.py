@@ -56,7 +56,7 @@ EXPORTS = (
     "swift_spmm_host", "swift_prepare_values", "swift_convert_dense", "swift_spmm_scaled",
     "swift_build_transpose", "swift_sddmm", "swift_backward_fused", "swift_update_values",
     "swift_batch_create", "swift_batch_spmm", "swift_batch_graph_launch",
-    "swift_batch_params", "swift_batch_destroy",
+    "swift_batch_params", "swift_batch_destroy", "swift_autotune",
 )
 
 _lib = None
@@ -117,6 +117,9 @@ def lib():
         L.swift_batch_graph_launch.argtypes = [_vp, _vp, _vp, _vp]
         L.swift_batch_params.argtypes = [_vp, _P(C.c_int), _P(C.c_int), _P(C.c_int)]
         L.swift_batch_destroy.argtypes = [_vp]
+        L.swift_batch_destroy.restype = None
+        L.swift_autotune.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
+                                     _P(C.c_int)]
         L.swift_batch_destroy.restype = None
         _lib = L
     return _lib
@@ -294,6 +297,16 @@ class SwiftMatrix:
             out[k] = arr
         return out
 
+    def autotune(self, B, out):
+        """swift_autotune on these operands (out is overwritten); returns the
+        chosen segment size."""
+        B = B.contiguous()
+        best = C.c_int(0)
+        check("swift_autotune", lib().swift_autotune(self.h, B.shape[1], _dtype_code(B.dtype),
+                                                     ROW_MAJOR, _ptr(B), _ptr(out), _stream(),
+                                                     C.byref(best)))
+        return best.value
+
     def set_segment_items(self, items):
         check("swift_set_segment_items", lib().swift_set_segment_items(self.h, items, _stream()))
         return self
@@ -416,6 +429,55 @@ def _hip():
         _hip_lib = C.CDLL("libamdhip64.so.7")
         _hip_lib.hipMemcpy.argtypes = [_vp, _vp, C.c_size_t, C.c_int]
     return _hip_lib
+
+
+# -------------------------------------------------------------- batched --
+
+class Batch:
+    """swift_batch_*: many FP32 row-major products C_i = A_i * B_i in one
+    persistent launch (optionally replayed from a captured HIP graph)."""
+
+    def __init__(self, mats, Ks):
+        self.mats = list(mats)
+        self.Ks = [int(k) for k in Ks]
+        n = len(self.mats)
+        handles = (_vp * n)(*[m.h for m in self.mats])
+        ks = (C.c_int * n)(*self.Ks)
+        h = _vp()
+        check("swift_batch_create", lib().swift_batch_create(C.byref(h), handles, ks, n,
+                                                             _stream()))
+        self.h = h
+
+    def _ptrs(self, Bs, Cs):
+        n = len(self.mats)
+        if len(Bs) != n or len(Cs) != n:
+            raise ValueError("need %d B and %d C tensors" % (n, n))
+        pb = (_vp * n)(*[_ptr(b) for b in Bs])
+        pc = (_vp * n)(*[_ptr(c) for c in Cs])
+        return pb, pc
+
+    def spmm(self, Bs, Cs, graph=False):
+        pb, pc = self._ptrs(Bs, Cs)
+        fn = "swift_batch_graph_launch" if graph else "swift_batch_spmm"
+        check(fn, getattr(lib(), fn)(self.h, pb, pc, _stream()))
+        return Cs
+
+    def params(self):
+        u, d, t = C.c_int(), C.c_int(), C.c_int()
+        check("swift_batch_params", lib().swift_batch_params(self.h, C.byref(u), C.byref(d),
+                                                             C.byref(t)))
+        return {"work_units": u.value, "depth": d.value, "block_threads": t.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().swift_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------- autograd --

@@ -1,0 +1,157 @@
+"""Batched persistent launch (swift_batch_*) and the autotuner: every product
+of the batch against the FP64 oracle (FP32 bound 1e-5 normalised by |A||B|)
+and against separate swift_spmm launches; graph replays checked one by one
+with eager work and new operands between them."""
+import numpy as np
+import pytest
+
+import gpu_common
+import matgen
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def make_batch(swift, count, seed=0, Ks=(32, 64, 128, 256)):
+    """Uneven FP32 matrices: generator items (square, power-law rows) at small
+    scale plus the edge-case matrices of the single-product tests."""
+    import torch
+    out = []
+    edge = gpu_common.cases()
+    names = ["empty_10x40", "uniform_64x64", "narrow_50x20", "wide_40x3000",
+             "densecol_400x100", "unsorted_rows_64x96", "tall_2000x64", "powerlaw_3000x2500"]
+    for i in range(count):
+        if i % 3 == 2:
+            m, n, rp, ci, v = edge[names[(i // 3) % len(names)]]
+        else:
+            d = swift.generate(5, scale=0.05, seed=seed, index=i)
+            m, n, rp, ci, v = d["m"], d["n"], d["rowptr"], d["colidx"], d["val"]
+        K = Ks[i % len(Ks)]
+        v32 = np.asarray(v, np.float32)
+        M = swift.SwiftMatrix(m, n, rp, ci, v32, dtype=torch.float32)
+        out.append((M, K, (m, n, rp, ci, v32)))
+    return out
+
+
+def operands(items, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    Bs, Bn, Cs = [], [], []
+    for M, K, (m, n, *_rest) in items:
+        b = rng.uniform(-1, 1, (n, K)).astype(np.float32)
+        Bn.append(b)
+        Bs.append(torch.tensor(b, device="cuda"))
+        Cs.append(torch.full((m, K), 7.0, device="cuda"))  # stale values must go
+    return Bs, Bn, Cs
+
+
+def check_all(oracle, items, Bn, Cs):
+    import torch
+    torch.cuda.synchronize()
+    for i, ((M, K, (m, n, rp, ci, v)), b, c) in enumerate(zip(items, Bn, Cs)):
+        C64, absC = oracle.spmm_f64(m, n, rp, ci, v.astype(np.float64), b.astype(np.float64))
+        err = gpu_common.rel_err(c.cpu().numpy(), C64, absC)
+        assert err <= TOL, "matrix %d (%dx%d, K=%d): %.3g" % (i, m, n, K, err)
+
+
+def test_batch_matches_oracle_and_separate(swift, oracle, cuda):
+    import torch
+    items = make_batch(swift, 24)
+    print("batch of", len(items), flush=True)
+    batch = swift.Batch([it[0] for it in items], [it[1] for it in items])
+    p = batch.params()
+    lo = sum(-(-(it[0].info().m + it[0].info().regular_nnz) // 512) for it in items)
+    hi = sum(-(-(it[0].info().m + it[0].info().regular_nnz) // 64) for it in items)
+    assert lo <= p["work_units"] <= hi
+    assert p["depth"] in (64, 128, 256, 512) and p["block_threads"] == 256
+    Bs, Bn, Cs = operands(items, 1)
+    batch.spmm(Bs, Cs)
+    check_all(oracle, items, Bn, Cs)
+    for (M, K, _), b, c in zip(items, Bs, Cs):
+        ref = M.spmm(b)
+        torch.cuda.synchronize()
+        assert torch.allclose(c, ref, rtol=1e-5, atol=1e-5)
+    batch.close()
+    for it in items:
+        it[0].close()
+
+
+def test_graph_replays(swift, oracle, cuda):
+    import torch
+    items = make_batch(swift, 12, seed=3)
+    batch = swift.Batch([it[0] for it in items], [it[1] for it in items])
+    Bs, Bn, Cs = operands(items, 2)
+    for rep in range(4):
+        print("replay", rep, flush=True)
+        for c in Cs:
+            c.fill_(-3.0)
+        batch.spmm(Bs, Cs, graph=True)
+        check_all(oracle, items, Bn, Cs)
+        # eager work between replays
+        x = torch.randn(1 << 20, device="cuda")
+        float(x.sum())
+    # new operand tensors: the descriptor table is rewritten, the graph kept
+    Bs2, Bn2, Cs2 = operands(items, 5)
+    batch.spmm(Bs2, Cs2, graph=True)
+    check_all(oracle, items, Bn2, Cs2)
+    batch.spmm(Bs, Cs, graph=False)
+    check_all(oracle, items, Bn, Cs)
+    batch.close()
+    for it in items:
+        it[0].close()
+
+
+def test_duplicate_matrix_in_batch(swift, oracle, cuda):
+    import torch
+    m, n, rp, ci, v = matgen.powerlaw_csr(700, 600, 21, max_len=300)
+    v32 = np.asarray(v, np.float32)
+    M = swift.SwiftMatrix(m, n, rp, ci, v32, dtype=torch.float32)
+    items = [(M, 32, (m, n, rp, ci, v32)), (M, 128, (m, n, rp, ci, v32))]
+    batch = swift.Batch([M, M], [32, 128])
+    Bs, Bn, Cs = operands(items, 9)
+    batch.spmm(Bs, Cs)
+    check_all(oracle, items, Bn, Cs)
+    batch.close()
+    M.close()
+
+
+def test_batch_rejects(swift, cuda):
+    import torch
+    m, n, rp, ci, v = matgen.random_csr(100, 80, 0.1, 4)
+    M = swift.SwiftMatrix(m, n, rp, ci, np.asarray(v, np.float32), dtype=torch.float32)
+    M64 = swift.SwiftMatrix(m, n, rp, ci, np.asarray(v, np.float64), dtype=torch.float64)
+    with pytest.raises(swift.SwiftError):
+        swift.Batch([M], [48])
+    with pytest.raises(swift.SwiftError):
+        swift.Batch([M64], [32])
+    batch = swift.Batch([M], [32])
+    B = torch.zeros((n * 32 + 1,), device="cuda")
+    C = torch.zeros((m * 32 + 1,), device="cuda")
+    with pytest.raises(swift.SwiftError):  # misaligned operands
+        batch.spmm([B[1:]], [C[1:]])
+    # re-segmenting a batched matrix invalidates the batch, loudly
+    M.set_segment_items(64 if batch.params()["depth"] != 64 else 128)
+    with pytest.raises(swift.SwiftError):
+        batch.spmm([B[:n * 32]], [C[:m * 32]])
+    batch.close()
+    M.close()
+    M64.close()
+
+
+def test_autotune_single(swift, oracle, cuda):
+    import torch
+    m, n, rp, ci, v = matgen.powerlaw_csr(3000, 2500, 6, max_len=400)
+    v32 = np.asarray(v, np.float32)
+    M = swift.SwiftMatrix(m, n, rp, ci, v32, dtype=torch.float32)
+    rng = np.random.default_rng(0)
+    b = rng.uniform(-1, 1, (n, 64)).astype(np.float32)
+    B = torch.tensor(b, device="cuda")
+    C = torch.empty((m, 64), device="cuda")
+    W = M.autotune(B, C)
+    assert W in (64, 128, 256, 512)
+    C = M.spmm(B)
+    torch.cuda.synchronize()
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, v32.astype(np.float64), b.astype(np.float64))
+    assert gpu_common.rel_err(C.cpu().numpy(), C64, absC) <= TOL
+    M.close()
