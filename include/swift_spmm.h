@@ -51,9 +51,13 @@ typedef enum {
     SWIFT_KERNEL_KCHUNK = 3,     /* 2-D grid tiles x K-chunks (Swift_GPU_3)                */
     SWIFT_KERNEL_ROWSORT = 4,    /* same-row segmented reduction over row-sorted groups
                                     of 8 tiles (Swift_GPU_4); needs swift_row_idx_sort */
-    SWIFT_KERNEL_ROWVIEW = 5     /* row-major: regular part walked output-stationary
-                                    through its row view (merge-path balanced), reside
-                                    part in balance blocks; the row-major default    */
+    SWIFT_KERNEL_ROWVIEW = 5,    /* row-major: regular part walked output-stationary
+                                    through its row view (merge-path balanced), one
+                                    row at a time per wave; reside part in balance
+                                    blocks                                           */
+    SWIFT_KERNEL_FLAT = 6        /* row-major: row view walked by independent lane
+                                    groups, several entries in flight whatever the
+                                    row lengths; reside part in balance blocks       */
 } swift_kernel;
 
 /* swift_spmm flags */

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_batch_zero(const RowArgs *__restrict__ 
 
 template <int LPE>
 __device__ __forceinline__ void batch_walk(const RowArgs &a, int sid) {
-    rowview_walk<float, float, float, 4, 1, LPE, false>(a, sid);
+    rowview_walk<float, float, float, 4, 1, LPE, false, false>(a, sid);
 }
 
 // Persistent: every wave takes items until the counter passes nitems (an exit

@@ -460,9 +460,10 @@ __global__ void k_row_hist(const int *__restrict__ rows, int n, int *__restrict_
 // original column of sorted column tile_colidx[p/32] + p%32.
 __global__ void k_rv_fill(int R, const unsigned *__restrict__ order,
                           const int *__restrict__ tile_colidx, const int *__restrict__ perm,
-                          const void *__restrict__ sval, const int *__restrict__ ssrc, int vs,
-                          int *__restrict__ rv_col, void *__restrict__ rv_val,
-                          int *__restrict__ rv_src) {
+                          const int *__restrict__ srow, const void *__restrict__ sval,
+                          const int *__restrict__ ssrc, int vs, int *__restrict__ rv_col,
+                          void *__restrict__ rv_val, int *__restrict__ rv_src,
+                          int *__restrict__ rv_row) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= R) return;
     int p = (int)order[q];
@@ -470,6 +471,17 @@ __global__ void k_rv_fill(int R, const unsigned *__restrict__ order,
     rv_col[q] = perm[c];
     copy_value(vs, rv_val, q, sval, p);
     rv_src[q] = ssrc[p];
+    rv_row[q] = srow[p];
+}
+// rows whose regular part is empty
+__global__ void k_empty_flag(int m, const int *__restrict__ ptr, int *__restrict__ flag) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m) flag[r] = ptr[r + 1] == ptr[r] ? 1 : 0;
+}
+__global__ void k_empty_compact(int m, const int *__restrict__ flag, const int *__restrict__ idx,
+                                int *__restrict__ out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m && flag[r]) out[idx[r]] = r;
 }
 
 __global__ void k_inv_perm(const int *__restrict__ perm, int n, int *__restrict__ inv) {
@@ -590,7 +602,8 @@ static void free_matrix_arrays(swift_matrix *M) {
                     M->blkstart, M->blkstop, M->blkptr, M->val_position, M->val_offset,
                     M->rowsorted, M->rowsort_src, M->csr_rowptr, M->csr_colidx,
                     M->sortval_lp, M->reside_val_lp, M->work, M->rv_ptr, M->rv_col,
-                    M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->seg, M->split_rows};
+                    M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->seg, M->split_rows,
+                    M->rv_row, M->empty_rows};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     M->scratch.release();
@@ -870,6 +883,7 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
     SWIFT_TRY(dalloc(&M->rv_ptr, (size_t)m + 1));
     SWIFT_TRY(dalloc(&M->rv_col, (size_t)R));
     SWIFT_TRY(dalloc(&M->rv_src, (size_t)R));
+    SWIFT_TRY(dalloc(&M->rv_row, (size_t)R));
     SWIFT_TRY(dalloc_bytes(&M->rv_val, (size_t)R * vs));
     SWIFT_TRY(dalloc(&M->inv_perm, (size_t)M->n));
     if (M->n) {
@@ -896,9 +910,28 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
                                    (size_t)R, bits_for((unsigned)(m > 0 ? m - 1 : 0)), tmp.ptr,
                                    s));
         hipLaunchKernelGGL(k_rv_fill, dim3(grid1(R)), dim3(256), 0, s, R, vals.as<unsigned>(),
-                           M->tile_colidx, M->perm, M->sortval, M->sortsrc, vs, M->rv_col,
-                           M->rv_val, M->rv_src);
+                           M->tile_colidx, M->perm, M->sortrowidx, M->sortval, M->sortsrc, vs,
+                           M->rv_col, M->rv_val, M->rv_src, M->rv_row);
         SWIFT_CHECK_LAUNCH();
+    }
+    // rows without regular entries
+    if (m) {
+        int *flag = cnt.as<int>();  // m+1 ints, free again
+        Buffer idx;
+        SWIFT_TRY(idx.alloc(sizeof(int) * ((size_t)m + 1)));
+        hipLaunchKernelGGL(k_empty_flag, dim3(grid1(m)), dim3(256), 0, s, m, M->rv_ptr, flag);
+        SWIFT_CHECK_LAUNCH();
+        SWIFT_TRY(exclusive_scan(flag, idx.as<int>(), (size_t)m, idx.as<int>() + m, tmp.ptr, s));
+        int ne = 0;
+        SWIFT_HIP(hipMemcpyAsync(&ne, idx.as<int>() + m, sizeof(int), hipMemcpyDeviceToHost, s));
+        SWIFT_HIP(hipStreamSynchronize(s));
+        SWIFT_TRY(dalloc(&M->empty_rows, (size_t)ne));
+        if (ne) {
+            hipLaunchKernelGGL(k_empty_compact, dim3(grid1(m)), dim3(256), 0, s, m, flag,
+                               idx.as<int>(), M->empty_rows);
+            SWIFT_CHECK_LAUNCH();
+        }
+        M->nempty = ne;
     }
     SWIFT_HIP(hipStreamSynchronize(s));
     return build_segments(M, 128, s);

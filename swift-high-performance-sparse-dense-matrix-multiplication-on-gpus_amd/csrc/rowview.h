@@ -73,6 +73,99 @@ __device__ __forceinline__ void load_b(const TB *__restrict__ p, TC (&o)[VEC]) {
     }
 }
 
+#ifndef SWIFT_RV_UNROLL
+#define SWIFT_RV_UNROLL 4
+#endif
+#ifndef SWIFT_RV_NT
+#define SWIFT_RV_NT 0
+#endif
+
+// Streaming (non-temporal) form of load_b for 16-byte FP32 and 8-byte 16-bit rows.
+template <class TB, int VEC, class TC>
+__device__ __forceinline__ void load_b_nt(const TB *__restrict__ p, TC (&o)[VEC]) {
+    if constexpr (std::is_same<TB, float>::value && VEC == 4) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(p));
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    } else if constexpr (std::is_same<TB, __hip_bfloat16>::value && VEC == 4) {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        u2 v = __builtin_nontemporal_load(reinterpret_cast<const u2 *>(p));
+        o[0] = __uint_as_float(v.x << 16);
+        o[1] = __uint_as_float(v.x & 0xFFFF0000u);
+        o[2] = __uint_as_float(v.y << 16);
+        o[3] = __uint_as_float(v.y & 0xFFFF0000u);
+    } else {
+        load_b<TB, VEC, TC>(p, o);
+    }
+}
+
+// Buffer-resource loads of B rows.  A raw buffer load whose byte offset is at
+// or past the resource's range returns zeros without touching memory, so an
+// unused slot takes the offset OOB_OFFSET instead of a branch around its load,
+// and a row offset is one 24-bit multiply-add (B rows < 2^24, B < 2^31 bytes).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+constexpr unsigned OOB_OFFSET = 0x80000000u;
+constexpr int GFX9_RSRC_WORD3 = 0x00020000;  // 32-bit data format, no swizzle
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             GFX9_RSRC_WORD3);
+}
+
+__device__ __forceinline__ float bf16_lo(unsigned w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(unsigned w) { return __uint_as_float(w & 0xFFFF0000u); }
+__device__ __forceinline__ float f16_lo(unsigned w) {
+    return __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu)));
+}
+__device__ __forceinline__ float f16_hi(unsigned w) {
+    return __half2float(__ushort_as_half((unsigned short)(w >> 16)));
+}
+
+template <class TB, class TC>
+__device__ __forceinline__ void widen_words(const unsigned *w, int nw, TC *o) {
+    if constexpr (std::is_same<TB, float>::value) {
+        for (int i = 0; i < nw; i++) o[i] = __uint_as_float(w[i]);
+    } else if constexpr (std::is_same<TB, double>::value) {
+        for (int i = 0; i < nw / 2; i++)
+            o[i] = __hiloint2double((int)w[2 * i + 1], (int)w[2 * i]);
+    } else if constexpr (std::is_same<TB, __hip_bfloat16>::value) {
+        for (int i = 0; i < nw; i++) {
+            o[2 * i] = bf16_lo(w[i]);
+            o[2 * i + 1] = bf16_hi(w[i]);
+        }
+    } else {
+        for (int i = 0; i < nw; i++) {
+            o[2 * i] = f16_lo(w[i]);
+            o[2 * i + 1] = f16_hi(w[i]);
+        }
+    }
+}
+
+// VEC consecutive elements of B at byte offset off of the resource.
+template <class TB, int VEC, class TC>
+__device__ __forceinline__ void load_b_buf(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                           TC (&o)[VEC]) {
+    constexpr int BYTES = (int)sizeof(TB) * VEC;
+    if constexpr (BYTES == 16) {
+        const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+        widen_words<TB, TC>(w, 4, o);
+    } else if constexpr (BYTES == 8) {
+        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        const unsigned w[2] = {v.x, v.y};
+        widen_words<TB, TC>(w, 2, o);
+    } else if constexpr (BYTES == 4) {
+        const unsigned w[1] = {__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0)};
+        widen_words<TB, TC>(w, 1, o);
+    } else {
+        static_assert(BYTES == 2, "unsupported B load width");
+        const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+        if constexpr (std::is_same<TB, __hip_bfloat16>::value) o[0] = bf16_lo(h);
+        else o[0] = f16_lo(h);
+    }
+}
+
 __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
 
@@ -103,6 +196,7 @@ struct RowArgs {
     const int2 *seg;
     const int *rv_ptr, *rv_col;
     const void *rv_val;
+    const int *rv_row;          // output row of each row-view entry
     const int *inv_perm;        // B in sorted order: B row of column c is inv_perm[c]
     int accumulate;
     // shared
@@ -114,7 +208,16 @@ struct RowArgs {
     int kbase, kcount;          // K slice handled by this launch
     const float *kscale;        // F16: per-k output scale (nullptr otherwise)
     float gscale;               // F16: A-value scale
+    // buffer-resource path (row-view walk): bytes of B from its K-slice start,
+    // and the row stride in bytes
+    unsigned bbytes, bstride;
 };
+
+// B can be read through a buffer resource with 24-bit row offsets: fewer than
+// 2^24 rows and fewer than 2^31 bytes.
+inline bool buffer_ok(long long nrows, long long ldb, size_t elem) {
+    return nrows < (1LL << 24) && nrows * ldb * (long long)elem < (1LL << 31);
+}
 
 template <class TC>
 __device__ __forceinline__ TC out_scale(const RowArgs &a, int k) {
@@ -212,6 +315,17 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks) {
     return x * per + min(x, rem) + b / X;
 }
 
+// Lane permute of a float / double value (ds_bpermute moves 32 bits).
+__device__ __forceinline__ float bperm_val(int addr, float x) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
+}
+__device__ __forceinline__ double bperm_val(int addr, double x) {
+    const long long u = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)u);
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(u >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Merge-path walk of the row view: each wave owns one segment of W items
 // (rows + entries), so every wave does the same amount of work whatever the
 // row lengths.  The segment's entries are contiguous in the row view, so they
@@ -222,10 +336,10 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks) {
 // row end the sub-groups' sums are combined with a butterfly and sub-group 0
 // stores the row.  Control flow is uniform across the wave.  Rows cut by a
 // segment boundary are split rows (zeroed beforehand) and take atomics.
-template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF>
 __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
     constexpr int EPS = 64 / LPE;
-    constexpr int UNROLL = 4;
+    constexpr int UNROLL = SWIFT_RV_UNROLL;
     constexpr int STEP = EPS * UNROLL;  // entries per iteration
     const int lane = threadIdx.x & 63;
     const int h = lane / LPE, sub = lane % LPE;
@@ -235,6 +349,8 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
     const bool carried_in = row < a.m && e > a.rv_ptr[row];
     const TV *val = static_cast<const TV *>(a.rv_val);
     const TB *B = static_cast<const TB *>(a.B);
+    __amdgpu_buffer_rsrc_t rsrc;
+    if constexpr (BUF) rsrc = make_rsrc(B + a.kbase, a.bbytes);
     // staged row ends: lane l holds the end of row rbase + l
     int rbase = row;
     int rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
@@ -274,6 +390,35 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
             for (int j = 0; j < nb; j += STEP) {
                 TC b[UNROLL][Q][VEC];
                 TC vv[UNROLL];
+                if constexpr (BUF) {
+                    // slot u reads staged lane off + j + u*EPS + h: one base
+                    // address per step, the slot in the instruction offset
+                    const int base = (off + j + h) << 2;
+#pragma unroll
+                    for (int u = 0; u < UNROLL; u++) {
+                        const int idx = j + u * EPS + h;
+                        const int c = __builtin_amdgcn_ds_bpermute(base + ((u * EPS) << 2), mc);
+                        const TC vs = bperm_val(base + ((u * EPS) << 2), mv);
+                        const bool ok = idx < nb;
+                        vv[u] = ok ? vs : (TC)0;
+                        const unsigned roff = __umul24((unsigned)c, a.bstride);
+#pragma unroll
+                        for (int q = 0; q < Q; q++) {
+                            const int k0 = q * LPE * VEC + sub * VEC;
+                            const bool in = ok && (!MASK || k0 < a.kcount);
+                            load_b_buf<TB, VEC, TC>(
+                                rsrc, in ? roff + (unsigned)(k0 * (int)sizeof(TB)) : OOB_OFFSET,
+                                b[u][q]);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNROLL; u++)
+#pragma unroll
+                        for (int q = 0; q < Q; q++)
+#pragma unroll
+                            for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * b[u][q][v];
+                    continue;
+                }
 #pragma unroll
                 for (int u = 0; u < UNROLL; u++) {
                     const int idx = j + u * EPS + h;
@@ -286,8 +431,10 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
                         const int k0 = q * LPE * VEC + sub * VEC;
-                        if (idx < nb && (!MASK || k0 < a.kcount))
-                            load_b<TB, VEC, TC>(brow + k0, b[u][q]);
+                        if (idx < nb && (!MASK || k0 < a.kcount)) {
+                            if (SWIFT_RV_NT) load_b_nt<TB, VEC, TC>(brow + k0, b[u][q]);
+                            else load_b<TB, VEC, TC>(brow + k0, b[u][q]);
+                        }
                         else
 #pragma unroll
                             for (int v = 0; v < VEC; v++) b[u][q][v] = (TC)0;
@@ -321,6 +468,105 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
         if (last) break;
         row++;
     }
+}
+
+// Flat walk of the row view: every group of LPE lanes is an independent
+// walker that owns one merge-path segment and takes its entries U at a time
+// whatever the row boundaries, so U B-row loads per walker (EPS*U per wave)
+// are in flight even where rows hold a single entry.  The segment's (column,
+// value, row) triples are staged LPE at a time in the walker's lanes (the
+// next batch is fetched one chunk early) and broadcast with shuffles; when
+// the row changes between two entries the finished row is written by the
+// walker alone (no cross-lane combine).  Only the loop over chunks is
+// wave-uniform; the row changes are per walker.  The first row of a segment
+// that started in an earlier segment and the last row if it continues into a
+// later one are split rows (zeroed beforehand) and take atomics; rows with no
+// regular entry are zeroed beforehand too.
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, int U>
+__device__ __forceinline__ void flat_walk(const RowArgs &a, int w) {
+    static_assert(U <= LPE, "a chunk must fit in one staged batch");
+    constexpr int SPAN = (LPE / U) * U;  // staged entries consumed per batch
+    const int lane = threadIdx.x & 63;
+    const int h = lane / LPE, sub = lane % LPE;
+    const bool live = w < a.nseg;
+    const int e0 = live ? a.seg[w].y : 0;
+    const int n = live ? a.seg[w + 1].y - e0 : 0;
+    const TV *val = static_cast<const TV *>(a.rv_val);
+    const TB *B = static_cast<const TB *>(a.B);
+    int mc = 0, mr = -1, nc = 0, nr = -1;
+    TC mv = (TC)0, nv = (TC)0;
+    auto stage = [&](int rel, int &c_, TC &v_, int &r_) {
+        c_ = 0;
+        v_ = (TC)0;
+        r_ = -1;
+        if (rel + sub < n) {
+            const int e = e0 + rel + sub;
+            const int c = a.rv_col[e];
+            c_ = a.inv_perm ? a.inv_perm[c] : c;
+            v_ = to_acc(val[e]);
+            r_ = a.rv_row[e];
+        }
+    };
+    stage(0, mc, mv, mr);
+    const int first = __shfl(mr, h * LPE, 64);  // row of the first entry (-1: none)
+    bool carried = false, cont = false;
+    if (n > 0) {
+        carried = e0 > a.rv_ptr[first];
+        const int lr = a.rv_row[e0 + n - 1];
+        cont = e0 + n < a.rv_ptr[lr + 1];
+    }
+    TC acc[Q][VEC];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+    int cur = first;
+    for (int rel = 0; __ballot(rel < n) != 0; rel += U) {
+        const int off = rel % SPAN;  // wave-uniform
+        if (off == 0 && rel > 0) {
+            mc = nc;
+            mv = nv;
+            mr = nr;
+        }
+        if (off + U == SPAN) stage(rel + U, nc, nv, nr);
+        TC b[U][Q][VEC];
+        TC vv[U];
+        int rr[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            // all lanes take part in the shuffles
+            const int src = h * LPE + off + u;
+            const int c = __shfl(mc, src, 64);
+            const TC vs = __shfl(mv, src, 64);
+            rr[u] = __shfl(mr, src, 64);
+            const bool ok = rel + u < n;
+            vv[u] = ok ? vs : (TC)0;
+            const TB *brow = B + (size_t)c * a.ldb + a.kbase;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int k0 = q * LPE * VEC + sub * VEC;
+                if (ok && (!MASK || k0 < a.kcount))
+                    load_b<TB, VEC, TC>(brow + k0, b[u][q]);
+                else
+#pragma unroll
+                    for (int v = 0; v < VEC; v++) b[u][q][v] = (TC)0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (rel + u < n) {
+                if (rr[u] != cur) {
+                    put_row<TC, VEC, Q, LPE, MASK>(a, cur, sub, cur == first && carried, acc);
+                    cur = rr[u];
+                }
+#pragma unroll
+                for (int q = 0; q < Q; q++)
+#pragma unroll
+                    for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * b[u][q][v];
+            }
+        }
+    }
+    if (n > 0) put_row<TC, VEC, Q, LPE, MASK>(a, cur, sub, (cur == first && carried) || cont, acc);
 }
 
 // One balance block (or one reside column when unbalanced) per wave: the

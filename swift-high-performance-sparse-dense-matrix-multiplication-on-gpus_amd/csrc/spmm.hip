@@ -76,12 +76,12 @@ __global__ __launch_bounds__(256) void k_tiles_row(RowArgs a) {
 }
 
 // Thin kernels over the device functions of rowview.h.
-template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF>
 __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
     const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
     const int sid = (int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6);
     if (sid >= a.nseg) return;
-    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK>(a, sid);
+    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF>(a, sid);
 }
 
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
@@ -89,6 +89,18 @@ __global__ __launch_bounds__(256) void k_reside_row(RowArgs a) {
     const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
     if (blk >= a.nblk) return;
     reside_block<TV, TB, TC, VEC, Q, LPE, MASK>(a, blk);
+}
+
+#ifndef SWIFT_FLAT_U
+#define SWIFT_FLAT_U 4
+#endif
+
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, int U>
+__global__ __launch_bounds__(256) void k_flat_row(RowArgs a) {
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int t = (int)(lb * (size_t)blockDim.x + threadIdx.x);
+    if ((t & ~63) / LPE >= a.nseg) return;  // every walker of this wave is past the end
+    flat_walk<TV, TB, TC, VEC, Q, LPE, MASK, U>(a, t / LPE);
 }
 
 // Zeroes C[row, kbase : kbase+kcount) of each split row; one wave per row.
@@ -305,14 +317,26 @@ static std::vector<std::pair<int, RowTarget>> plan_row_slices(int K, int vec, bo
     return out;
 }
 
-enum RowKind { ROW_TILES = 0, ROW_RESIDE = 1, ROW_VIEW = 2 };
+enum RowKind { ROW_TILES = 0, ROW_RESIDE = 1, ROW_VIEW = 2, ROW_FLAT = 3 };
 
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
 static void launch_row_shape(const RowArgs &a, RowKind kind, hipStream_t s) {
-    if (kind == ROW_VIEW) {
+    if (kind == ROW_FLAT) {
+        constexpr int U = SWIFT_FLAT_U < LPE ? SWIFT_FLAT_U : LPE;
         if (a.nseg)
-            hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK>),
-                               dim3(ceil_div((size_t)a.nseg * 64, 256)), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_flat_row<TV, TB, TC, VEC, Q, LPE, MASK, U>),
+                               dim3(ceil_div((size_t)a.nseg * LPE, 256)), dim3(256), 0, s, a);
+        return;
+    }
+    if (kind == ROW_VIEW) {
+        if (!a.nseg) return;
+        const dim3 grid(ceil_div((size_t)a.nseg * 64, 256));
+        if (a.bstride)
+            hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, true>), grid,
+                               dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, false>), grid,
+                               dim3(256), 0, s, a);
         return;
     }
     const unsigned items = kind == ROW_TILES ? (unsigned)a.nwork : (unsigned)a.nblk;
@@ -377,6 +401,7 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.rv_ptr = M->rv_ptr;
     a.rv_col = M->rv_col;
     a.rv_val = rv_val;
+    a.rv_row = M->rv_row;
     const bool sorted = (flags & SWIFT_B_SORTED) != 0;
     a.inv_perm = sorted ? M->inv_perm : nullptr;
     a.accumulate = (flags & SWIFT_ACCUMULATE) ? 1 : 0;
@@ -391,22 +416,31 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     constexpr int V = Vec16<TB>::N;
     const bool aligned = (K % V) == 0 && ((uintptr_t)B % 16) == 0 && ((uintptr_t)C % 16) == 0;
     const auto slices = plan_row_slices(K, V, aligned);
-    const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW;
+    const bool flat = kernel == SWIFT_KERNEL_FLAT;
+    const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW || flat;
     if (view) {
         if (!a.accumulate && M->nsplit) {
             hipLaunchKernelGGL(k_zero_rows<TC>, dim3(ceil_div((size_t)M->nsplit * 64, 256)),
                                dim3(256), 0, s, M->split_rows, M->nsplit, (TC *)C, K, K);
             SWIFT_CHECK_LAUNCH();
         }
+        if (flat && !a.accumulate && M->nempty) {
+            hipLaunchKernelGGL(k_zero_rows<TC>, dim3(ceil_div((size_t)M->nempty * 64, 256)),
+                               dim3(256), 0, s, M->empty_rows, M->nempty, (TC *)C, K, K);
+            SWIFT_CHECK_LAUNCH();
+        }
     } else if (!a.accumulate) {
         SWIFT_TRY(fill_bytes_zero(C, (size_t)M->m * K * sizeof(TC), s));
     }
     mark(1, s);
+    const bool buf = buffer_ok(M->n, K, sizeof(TB));
     for (auto &sl : slices) {
         a.kbase = sl.first;
         const RowTarget &t = sl.second;
         a.kcount = t.mask ? std::min(64, K - sl.first) : t.vec * t.lpe * t.q;
-        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, view ? ROW_VIEW : ROW_TILES, s)));
+        a.bstride = buf ? (unsigned)(K * sizeof(TB)) : 0u;
+        a.bbytes = buf ? (unsigned)(((size_t)M->n * K - sl.first) * sizeof(TB)) : 0u;
+        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, flat ? ROW_FLAT : view ? ROW_VIEW : ROW_TILES, s)));
     }
     mark(2, s);
     for (auto &sl : slices) {
@@ -583,9 +617,14 @@ int spmm_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dty
         set_error("SWIFT_KERNEL_ROWSORT is a column-major kernel");
         return SWIFT_ERR_UNSUPPORTED;
     }
-    if (kernel == SWIFT_KERNEL_ROWVIEW && layout == SWIFT_COL_MAJOR) {
-        set_error("SWIFT_KERNEL_ROWVIEW is a row-major kernel");
+    if ((kernel == SWIFT_KERNEL_ROWVIEW || kernel == SWIFT_KERNEL_FLAT) &&
+        layout == SWIFT_COL_MAJOR) {
+        set_error("SWIFT_KERNEL_ROWVIEW / SWIFT_KERNEL_FLAT are row-major kernels");
         return SWIFT_ERR_UNSUPPORTED;
+    }
+    if ((int)kernel < 0 || (int)kernel > (int)SWIFT_KERNEL_FLAT) {
+        set_error("swift_spmm: unknown kernel %d", (int)kernel);
+        return SWIFT_ERR_ARG;
     }
     mark(0, s);
     const void *sv = lowp ? M->sortval_lp : M->sortval;

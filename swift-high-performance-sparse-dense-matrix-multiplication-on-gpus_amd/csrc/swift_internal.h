@@ -154,6 +154,9 @@ struct swift_matrix {
     int *rv_col = nullptr;       // B row (original column) of each entry
     void *rv_val = nullptr;      // value (vtype)
     int *rv_src = nullptr;       // CSR position of each entry
+    int *rv_row = nullptr;       // output row of each entry
+    int *empty_rows = nullptr;   // rows with no regular entry (zeroed before a flat walk)
+    int nempty = 0;
     void *rv_val_lp = nullptr;   // BF16/F16 copy (swift_prepare_values)
     int *inv_perm = nullptr;     // inv_perm[perm[i]] = i
     // merge-path split of (rows + regular entries) into segments of mp_items

@@ -30,15 +30,16 @@ def setup(swift, oracle, name, K, seed=0):
 @pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "equalrows_100x70",
                                   "narrow_50x20"])
 @pytest.mark.parametrize("K", [32, 128, 256])
-@pytest.mark.parametrize("layout", ["ROW", "COL"])
+@pytest.mark.parametrize("layout", ["ROW", "COL", "ROW_FLAT"])
 def test_bf16_storage(swift, oracle, cuda, name, K, layout):
     import torch
     M, B, C64, absC = setup(swift, oracle, name, K)
-    lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
+    lay = swift.COL_MAJOR if layout == "COL" else swift.ROW_MAJOR
+    kernel = swift.KERNEL_FLAT if layout == "ROW_FLAT" else swift.KERNEL_DEFAULT
     M.prepare_values(swift.BF16)
     Bt = torch.tensor(B if lay == swift.ROW_MAJOR else B.T.copy(), device="cuda")
     Bh = swift.convert_dense(Bt, swift.BF16, layout=lay)
-    C = M.spmm(Bh, layout=lay, dtype=swift.BF16).cpu().numpy()
+    C = M.spmm(Bh, layout=lay, dtype=swift.BF16, kernel=kernel).cpu().numpy()
     if lay == swift.COL_MAJOR:
         C = C.T
     err = gpu_common.rel_err(C, C64, absC)
@@ -50,11 +51,12 @@ def test_bf16_storage(swift, oracle, cuda, name, K, layout):
 
 @pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100"])
 @pytest.mark.parametrize("K", [32, 128])
-@pytest.mark.parametrize("layout", ["ROW", "COL"])
+@pytest.mark.parametrize("layout", ["ROW", "COL", "ROW_FLAT"])
 def test_f16_scaled_storage(swift, oracle, cuda, name, K, layout):
     import torch
     M, B, C64, absC = setup(swift, oracle, name, K)
-    lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
+    lay = swift.COL_MAJOR if layout == "COL" else swift.ROW_MAJOR
+    kernel = swift.KERNEL_FLAT if layout == "ROW_FLAT" else swift.KERNEL_DEFAULT
     # widen the dynamic range so that the per-column scales matter
     B = B * np.logspace(-3, 3, K, dtype=np.float32)[None, :]
     C64, absC = oracle.spmm_f64(*CASES[name][:4], np.asarray(CASES[name][4], np.float32)
@@ -62,7 +64,7 @@ def test_f16_scaled_storage(swift, oracle, cuda, name, K, layout):
     M.prepare_values(swift.F16)
     Bt = torch.tensor(B if lay == swift.ROW_MAJOR else B.T.copy(), device="cuda")
     Bh, sc = swift.convert_dense(Bt, swift.F16, layout=lay, col_scale=True)
-    C = M.spmm_scaled(Bh, sc, layout=lay).cpu().numpy()
+    C = M.spmm_scaled(Bh, sc, layout=lay, kernel=kernel).cpu().numpy()
     if lay == swift.COL_MAJOR:
         C = C.T
     err = gpu_common.rel_err(C, C64, absC)

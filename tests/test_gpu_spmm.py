@@ -55,7 +55,7 @@ def test_col_major(swift, oracle, cuda, name, K, dt):
     assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
 
 
-@pytest.mark.parametrize("kernel", ["BALANCED", "UNBALANCED", "KCHUNK", "ROWVIEW"])
+@pytest.mark.parametrize("kernel", ["BALANCED", "UNBALANCED", "KCHUNK", "ROWVIEW", "FLAT"])
 @pytest.mark.parametrize("layout", ["ROW", "COL"])
 @pytest.mark.parametrize("name", ["densecol_400x100", "equalrows_100x70", "powerlaw_3000x2500",
                                   "empty_10x40", "uniform_64x64"])
@@ -63,8 +63,8 @@ def test_kernel_variants(swift, oracle, cuda, kernel, layout, name):
     print("case", name, kernel, layout, flush=True)
     if layout == "ROW" and kernel == "KCHUNK":
         pytest.skip("K chunks are a column-major grid dimension")
-    if layout == "COL" and kernel == "ROWVIEW":
-        pytest.skip("the row view is a row-major kernel")
+    if layout == "COL" and kernel in ("ROWVIEW", "FLAT"):
+        pytest.skip("the row-view walks are row-major kernels")
     k = getattr(swift, "KERNEL_" + kernel)
     lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
     C, C64, absC = run_case(swift, oracle, name, 64, "f32", lay, k)
@@ -112,11 +112,13 @@ def test_b_sorted_flag(swift, oracle, cuda, layout, dt):
     assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
 
 
-@pytest.mark.parametrize("kernel", ["DEFAULT", "BALANCED"])
+@pytest.mark.parametrize("kernel", ["DEFAULT", "BALANCED", "ROWVIEW", "FLAT"])
 @pytest.mark.parametrize("K", [32, 100])
 @pytest.mark.parametrize("layout", ["ROW", "COL"])
 def test_accumulate_flag(swift, oracle, cuda, kernel, K, layout):
     import torch
+    if layout == "COL" and kernel in ("ROWVIEW", "FLAT"):
+        pytest.skip("the row-view walks are row-major kernels")
     m, n, rp, ci, v = CASES["powerlaw_3000x2500"]
     vv = np.asarray(v, np.float32)
     B = np.random.default_rng(1).uniform(-1, 1, (n, K)).astype(np.float32)
@@ -184,9 +186,55 @@ def test_rowview_col_major_rejected(swift, cuda):
     import torch
     m, n, rp, ci, v = CASES["random_300x200"]
     M = swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float32)
+    for k in (swift.KERNEL_ROWVIEW, swift.KERNEL_FLAT):
+        with pytest.raises(swift.SwiftError):
+            M.spmm(torch.zeros((32, n), device="cuda"), layout=swift.COL_MAJOR, kernel=k)
     with pytest.raises(swift.SwiftError):
-        M.spmm(torch.zeros((32, n), device="cuda"), layout=swift.COL_MAJOR,
-               kernel=swift.KERNEL_ROWVIEW)
+        M.spmm(torch.zeros((n, 32), device="cuda"), kernel=99)
+    M.close()
+
+
+@pytest.mark.parametrize("W", [8, 24, 128, 512])
+@pytest.mark.parametrize("K", [32, 100, 128, 256])
+@pytest.mark.parametrize("name", list(CASES))
+def test_flat_walk(swift, oracle, cuda, name, K, W):
+    """Lane-group walkers over segments of W items (W = 8 splits almost every
+    row, leaves walkers without entries); C starts as NaN so a row that is not
+    written (empty, split or stored) shows up."""
+    import torch
+    print("case", name, K, W, flush=True)
+    m, n, rp, ci, v = CASES[name]
+    vv = np.asarray(v, np.float32)
+    B = np.random.default_rng(K + W).uniform(-1, 1, (n, K)).astype(np.float32)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=torch.float32)
+    M.set_segment_items(W)
+    out = torch.full((m, K), float("nan"), device="cuda")
+    M.spmm(torch.tensor(B, device="cuda"), out=out, kernel=swift.KERNEL_FLAT)
+    C = out.cpu().numpy()
+    assert not np.isnan(C).any()
+    assert gpu_common.rel_err(C, C64, absC) <= 1e-5
+    M.close()
+
+
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "unsorted_rows_64x96", "empty_10x40"])
+@pytest.mark.parametrize("K", [8, 64, 100])
+def test_flat_walk_f64_sorted(swift, oracle, cuda, name, K):
+    import torch
+    m, n, rp, ci, v = CASES[name]
+    vv = np.asarray(v, np.float64)
+    B = np.random.default_rng(K).uniform(-1, 1, (n, K))
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv, B)
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=torch.float64)
+    M.set_segment_items(24)
+    perm = torch.tensor(M.arrays(["perm"])["perm"].astype(np.int64), device="cuda")
+    Bt = torch.tensor(B, device="cuda")
+    out = torch.full((m, K), float("nan"), device="cuda", dtype=torch.float64)
+    M.spmm(Bt, out=out, kernel=swift.KERNEL_FLAT)
+    assert gpu_common.rel_err(out.cpu().numpy(), C64, absC) <= 1e-12
+    out.fill_(float("nan"))
+    M.spmm(Bt[perm].contiguous(), out=out, kernel=swift.KERNEL_FLAT, flags=swift.B_SORTED)
+    assert gpu_common.rel_err(out.cpu().numpy(), C64, absC) <= 1e-12
     M.close()
 
 
