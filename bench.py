@@ -39,6 +39,19 @@ def algorithmic_bytes(m, n, nnz, K, s_a=4, s_b=4, s_c=4):
     return nnz * (4 + s_a) + 4 * (m + 1) + n * K * s_b + m * K * s_c
 
 
+def measured_traffic(path):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy calibration)."""
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 2 and f[0] == "traffic_bytes":
+                return int(f[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(d, K, seed):
     """Reference Swift_cpu (src/Swift_cpu.h:31-65), 1 thread, on the same matrix."""
     import oracle as O
@@ -76,6 +89,8 @@ def main():
     ap.add_argument("--extras", action="store_true", help="also time the other forms")
     ap.add_argument("--batch", type=int, default=0,
                     help="also time the batched form over this many generated matrices")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a plain 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the Swift SpMM path runs on a single MI355X "
@@ -87,6 +102,13 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.init()
+    if args.calibrate:
+        # known traffic for hardware-counter scaling: 256 MiB read + 256 MiB written
+        src = torch.ones(64 << 20, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     K, seed = args.K, 0
     t0 = time.time()
     d = S.generate(2, scale=args.scale, seed=seed)
@@ -134,6 +156,10 @@ def main():
     log("phases (median ms): zero %.4f regular %.4f reside %.4f" % (zero_ms, reg_ms, res_ms))
     alg = algorithmic_bytes(m, n, nnz, K)
     achieved = alg / (reg_ms * 1e-3)
+    # the committed counter summary covers the default workload only
+    default = args.scale == 1.0 and K == 128
+    traffic = measured_traffic(os.path.join(
+        ROOT, "profiles", "rowview_f32_k128_1Mx1M_16Mnnz_traffic.txt")) if default else None
     out = {
         "metric": "SpMM GFLOP/s + % HBM roofline (K=128 fp32); bf16 & fwd+bwd vs plain fp32",
         "value": round(value, 2),
@@ -156,7 +182,9 @@ def main():
                                 "reside": round(res_ms, 4)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "traffic_frac": (round(traffic / (reg_ms * 1e-3) / HBM_PEAK, 4)
+                                      if traffic else None),
                      "kernel": "k_rowview_row (regular part, row view)",
                      "algorithmic_bytes": alg},
         "gpu": torch.cuda.get_device_name(0),

@@ -176,7 +176,9 @@ extern "C" int swift_set_segment_items(swift_matrix *mat, int items, swift_strea
         set_error("swift_set_segment_items: items must be in [8, 2^20]");
         return SWIFT_ERR_ARG;
     }
-    SWIFT_GUARD(return build_segments(mat, items, S(stream));)
+    SWIFT_GUARD(int rc = build_segments(mat, items, S(stream));
+                if (rc == SWIFT_OK) mat->user_items = items;
+                return rc;)
 }
 
 extern "C" int swift_get_info(const swift_matrix *M, swift_info *o) {

@@ -7,7 +7,7 @@
 // The fused pass walks the transposed Swift layout (built once by
 // swift_build_transpose): each dC row is gathered once per nonzero and feeds
 // both dB and dvals, and dB rows are written once, not scattered.
-#include "swift_internal.h"
+#include "rowview.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -169,6 +169,7 @@ struct BwArgs {
     void *dvals;               // CSR-order output or nullptr
     const int *split_rows;
     int nsplit;
+    unsigned gbytes, gstride;  // buffer-resource path for G (0: plain loads)
 };
 
 template <class T> struct V16;
@@ -214,7 +215,7 @@ __device__ __forceinline__ T group_sum(T v) {
     return v;
 }
 
-template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
+template <class T, int VEC, int Q, int LPE, bool DB, bool DV, bool BUF>
 __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     constexpr int EPS = 64 / LPE;
     constexpr int UNROLL = 4;
@@ -231,6 +232,8 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     const T *val = static_cast<const T *>(a.rv_val);
     const T *G = static_cast<const T *>(a.G);
     const T *H = static_cast<const T *>(a.H);
+    __amdgpu_buffer_rsrc_t rsrc;
+    if constexpr (BUF) rsrc = make_rsrc(G, a.gbytes);
     // staged row ends and entries, as in the forward row-view kernel
     int rbase = row;
     int rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
@@ -274,8 +277,30 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 T g[UNROLL][Q][VEC];
                 T vv[UNROLL];
                 int src[UNROLL];
+                if constexpr (BUF) {
+                    // as in the forward walk: one permute base per step, G rows
+                    // by 24-bit offsets, unused slots read zeros out of range
+                    const int base = (off + j + h) << 2;
 #pragma unroll
-                for (int u = 0; u < UNROLL; u++) {
+                    for (int u = 0; u < UNROLL; u++) {
+                        const int addr = base + ((u * EPS) << 2);
+                        const int c = __builtin_amdgcn_ds_bpermute(addr, mc);
+                        src[u] = __builtin_amdgcn_ds_bpermute(addr, ms);
+                        const T vs = bperm_val(addr, mv);
+                        const bool ok = j + u * EPS + h < nb;
+                        vv[u] = ok ? vs : (T)0;
+                        const unsigned roff = __umul24((unsigned)c, a.gstride);
+#pragma unroll
+                        for (int q = 0; q < Q; q++)
+                            load_b_buf<T, VEC, T>(
+                                rsrc,
+                                ok ? roff + (unsigned)((q * LPE * VEC + sub * VEC) * (int)sizeof(T))
+                                   : OOB_OFFSET,
+                                g[u][q]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL && !BUF; u++) {
                     const int idx = j + u * EPS + h;
                     // unconditional shuffles: a lane reading a masked-off source
                     // lane would get a stale value
@@ -414,9 +439,14 @@ static int launch_bw(const BwArgs &a, hipStream_t s) {
     if (DB && a.nsplit)
         hipLaunchKernelGGL(k_zero_rows_bw<T>, dim3(ceil_div((size_t)a.nsplit * 64, 256)), dim3(256),
                            0, s, a.split_rows, a.nsplit, (T *)a.dB, a.K);
-    if (a.nseg)
-        hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, DB, DV>),
-                           dim3(ceil_div((size_t)a.nseg * 64, 256)), dim3(256), 0, s, a);
+    if (a.nseg) {
+        const dim3 grid(ceil_div((size_t)a.nseg * 64, 256));
+        if (a.gstride)
+            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, DB, DV, true>), grid, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, DB, DV, false>), grid, dim3(256), 0, s,
+                               a);
+    }
     if (a.nblk)
         hipLaunchKernelGGL((k_bw_reside<T, VEC, Q, LPE, DB, DV>),
                            dim3(ceil_div((size_t)a.nblk * 64, 256)), dim3(256), 0, s, a);
@@ -470,6 +500,14 @@ static BwArgs walk_args(const swift_matrix *W, int K) {
     return a;
 }
 
+// G is gathered by the walked matrix's column index: W->n rows of K.
+template <class T>
+static void set_g_buffer(BwArgs &a, const swift_matrix *W, int K) {
+    const bool ok = buffer_ok(W->n, K, sizeof(T));
+    a.gstride = ok ? (unsigned)(K * sizeof(T)) : 0u;
+    a.gbytes = ok ? (unsigned)((size_t)W->n * K * sizeof(T)) : 0u;
+}
+
 // dvals = SDDMM(dC, B) walking A: H = dC (per row), G = B (per entry).
 int sddmm_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, const void *B,
                void *dvals, hipStream_t s) {
@@ -481,6 +519,8 @@ int sddmm_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, 
     a.H = dC;
     a.G = B;
     a.dvals = dvals;
+    if (dtype == SWIFT_F64) set_g_buffer<double>(a, A, K);
+    else set_g_buffer<float>(a, A, K);
     return dtype == SWIFT_F64 ? dispatch_bw<double, false, true>(a, s)
                               : dispatch_bw<float, false, true>(a, s);
 }
@@ -499,6 +539,8 @@ int backward_impl(const swift_matrix *AT, int K, swift_dtype dtype, const void *
     a.dB = dB;
     a.dvals = dvals;
     const bool f64 = dtype == SWIFT_F64;
+    if (f64) set_g_buffer<double>(a, AT, K);
+    else set_g_buffer<float>(a, AT, K);
     if (dB && dvals) return f64 ? dispatch_bw<double, true, true>(a, s) : dispatch_bw<float, true, true>(a, s);
     if (dB) return f64 ? dispatch_bw<double, true, false>(a, s) : dispatch_bw<float, true, false>(a, s);
     return f64 ? dispatch_bw<double, false, true>(a, s) : dispatch_bw<float, false, true>(a, s);

@@ -47,13 +47,14 @@ __global__ __launch_bounds__(256) void k_batch_zero(const RowArgs *__restrict__ 
     for (int k = threadIdx.x & 63; k < K; k += 64) c[k] = 0.0f;
 }
 
-template <int LPE>
+template <int LPE, bool BUF>
 __device__ __forceinline__ void batch_walk(const RowArgs &a, int sid) {
-    rowview_walk<float, float, float, 4, 1, LPE, false, false>(a, sid);
+    rowview_walk<float, float, float, 4, 1, LPE, false, BUF>(a, sid);
 }
 
 // Persistent: every wave takes items until the counter passes nitems (an exit
-// every wave reaches).
+// every wave reaches).  BUF: every matrix's B fits the buffer-resource path.
+template <bool BUF>
 __global__ __launch_bounds__(256) void k_batch_walk(const RowArgs *__restrict__ desc,
                                                     const int2 *__restrict__ items, int nitems,
                                                     int *__restrict__ counter) {
@@ -67,10 +68,10 @@ __global__ __launch_bounds__(256) void k_batch_walk(const RowArgs *__restrict__ 
         const int sid = __builtin_amdgcn_readfirstlane(w.y);
         const RowArgs a = desc[mi];
         switch (a.ldb) {
-            case 32: batch_walk<8>(a, sid); break;
-            case 64: batch_walk<16>(a, sid); break;
-            case 128: batch_walk<32>(a, sid); break;
-            default: batch_walk<64>(a, sid); break;  // 256
+            case 32: batch_walk<8, BUF>(a, sid); break;
+            case 64: batch_walk<16, BUF>(a, sid); break;
+            case 128: batch_walk<32, BUF>(a, sid); break;
+            default: batch_walk<64, BUF>(a, sid); break;  // 256
         }
     }
 }
@@ -118,6 +119,7 @@ int autotune_impl(swift_matrix *M, int K, swift_dtype dtype, swift_layout layout
     int rc = SWIFT_OK;
     for (int W : cands) {
         if ((rc = build_segments(M, W, s)) != SWIFT_OK) break;
+        M->user_items = W;  // time this table, not the narrow-row one
         std::vector<float> t;
         for (int r = 0; r < 4 && rc == SWIFT_OK; r++) {
             (void)hipEventRecord(e0, s);
@@ -139,6 +141,7 @@ int autotune_impl(swift_matrix *M, int K, swift_dtype dtype, swift_layout layout
     (void)hipEventDestroy(e1);
     if (rc != SWIFT_OK) return rc;
     SWIFT_TRY(build_segments(M, best, s));
+    M->user_items = best;
     if (best_out) *best_out = best;
     return SWIFT_OK;
 }
@@ -156,6 +159,7 @@ struct swift_batch {
     int nitems = 0, nres = 0, nsplit = 0;
     int *counter = nullptr;
     int grid = 0;
+    bool buf = true;  // all B operands fit the buffer-resource path
     std::vector<const float *> lastB;
     std::vector<float *> lastC;
     hipGraph_t graph = nullptr;
@@ -195,6 +199,10 @@ static RowArgs row_args_of(const swift_matrix *M, int K) {
     a.kbase = 0;
     a.kcount = K;
     a.gscale = 1.0f;
+    if (buffer_ok(M->n, K, sizeof(float))) {
+        a.bstride = (unsigned)(K * sizeof(float));
+        a.bbytes = (unsigned)((size_t)M->n * K * sizeof(float));
+    }
     return a;
 }
 
@@ -258,6 +266,7 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
             }
             for (int i = 0; i < count && rc == SWIFT_OK; i++)
                 if (mats[i]->mp_items != b->W[i]) rc = build_segments(mats[i], b->W[i], s);
+            for (int i = 0; i < count && rc == SWIFT_OK; i++) mats[i]->user_items = b->W[i];
         }
         if (rc == SWIFT_OK) {
             // flattened work lists (host) -> device
@@ -277,6 +286,7 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
                 }
                 for (int r : rows) split.push_back(make_int2(i, r));
                 b->desc_h.push_back(row_args_of(M, b->Ks[i]));
+                if (!b->desc_h.back().bstride) b->buf = false;
             }
             auto up = [&](void **dst, const void *src, size_t bytes) -> int {
                 if (hipMalloc(dst, std::max<size_t>(bytes, 16)) != hipSuccess) {
@@ -361,9 +371,14 @@ static int batch_launch(swift_batch *b, hipStream_t s) {
     if (b->nsplit)
         hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div((size_t)b->nsplit * 64, 256)), dim3(256), 0,
                            s, b->desc, b->split, b->nsplit);
-    if (b->nitems)
-        hipLaunchKernelGGL(k_batch_walk, dim3(b->grid), dim3(256), 0, s, b->desc, b->items,
-                           b->nitems, b->counter);
+    if (b->nitems) {
+        if (b->buf)
+            hipLaunchKernelGGL(k_batch_walk<true>, dim3(b->grid), dim3(256), 0, s, b->desc,
+                               b->items, b->nitems, b->counter);
+        else
+            hipLaunchKernelGGL(k_batch_walk<false>, dim3(b->grid), dim3(256), 0, s, b->desc,
+                               b->items, b->nitems, b->counter);
+    }
     if (b->nres)
         hipLaunchKernelGGL(k_batch_reside, dim3(ceil_div((size_t)b->nres * 64, 256)), dim3(256), 0,
                            s, b->desc, b->res_items, b->nres);

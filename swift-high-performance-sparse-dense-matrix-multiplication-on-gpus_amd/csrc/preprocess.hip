@@ -603,7 +603,7 @@ static void free_matrix_arrays(swift_matrix *M) {
                     M->rowsorted, M->rowsort_src, M->csr_rowptr, M->csr_colidx,
                     M->sortval_lp, M->reside_val_lp, M->work, M->rv_ptr, M->rv_col,
                     M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->seg, M->split_rows,
-                    M->rv_row, M->empty_rows};
+                    M->rv_row, M->empty_rows, M->alt_seg, M->alt_split_rows};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     M->scratch.release();
@@ -839,40 +839,56 @@ int build_impl(swift_matrix *M, const int *d_rowptr, const int *d_colidx,
     return SWIFT_OK;
 }
 
-int build_segments(swift_matrix *M, int W, hipStream_t s) {
-    if (M->seg) { (void)hipFree(M->seg); M->seg = nullptr; }
-    if (M->split_rows) { (void)hipFree(M->split_rows); M->split_rows = nullptr; }
+// Merge-path segment table of W items: (row, entry) start of every segment
+// and the rows cut by a segment boundary.
+static int build_seg_table(swift_matrix *M, int W, hipStream_t s, int2 **seg_out, int *nseg_out,
+                           int **split_out, int *nsplit_out) {
+    if (*seg_out) { (void)hipFree(*seg_out); *seg_out = nullptr; }
+    if (*split_out) { (void)hipFree(*split_out); *split_out = nullptr; }
     const long long items = (long long)M->m + M->regular_nnz;
     const int nseg = (int)((items + W - 1) / W);
-    M->mp_items = W;
-    M->nseg = nseg;
-    M->nsplit = 0;
-    SWIFT_TRY(dalloc(&M->seg, (size_t)nseg + 1));
+    *nseg_out = nseg;
+    *nsplit_out = 0;
+    SWIFT_TRY(dalloc(seg_out, (size_t)nseg + 1));
+    int2 *seg = *seg_out;
     hipLaunchKernelGGL(k_mp_seg, dim3(grid1((size_t)nseg + 1)), dim3(256), 0, s, M->m,
-                       M->regular_nnz, W, nseg, M->rv_ptr, M->seg);
+                       M->regular_nnz, W, nseg, M->rv_ptr, seg);
     SWIFT_CHECK_LAUNCH();
-    if (nseg == 0) return SWIFT_OK;
+    if (nseg == 0) return dalloc(split_out, 1);
     Buffer flag, idx, tmp;
     SWIFT_TRY(flag.alloc(sizeof(int) * ((size_t)nseg + 1)));
     SWIFT_TRY(idx.alloc(sizeof(int) * ((size_t)nseg + 1)));
     SWIFT_TRY(tmp.alloc(scan_temp_bytes((size_t)nseg)));
     hipLaunchKernelGGL(k_mp_split_flag, dim3(grid1(nseg)), dim3(256), 0, s, M->m, nseg, M->rv_ptr,
-                       M->seg, flag.as<int>());
+                       seg, flag.as<int>());
     SWIFT_CHECK_LAUNCH();
     SWIFT_TRY(exclusive_scan(flag.as<int>(), idx.as<int>(), (size_t)nseg, idx.as<int>() + nseg,
                              tmp.ptr, s));
     int nsplit = 0;
     SWIFT_HIP(hipMemcpyAsync(&nsplit, idx.as<int>() + nseg, sizeof(int), hipMemcpyDeviceToHost, s));
     SWIFT_HIP(hipStreamSynchronize(s));
-    SWIFT_TRY(dalloc(&M->split_rows, (size_t)nsplit));
+    SWIFT_TRY(dalloc(split_out, (size_t)nsplit));
     if (nsplit) {
-        hipLaunchKernelGGL(k_mp_split_compact, dim3(grid1(nseg)), dim3(256), 0, s, nseg, M->seg,
-                           flag.as<int>(), idx.as<int>(), M->split_rows);
+        hipLaunchKernelGGL(k_mp_split_compact, dim3(grid1(nseg)), dim3(256), 0, s, nseg, seg,
+                           flag.as<int>(), idx.as<int>(), *split_out);
         SWIFT_CHECK_LAUNCH();
     }
     SWIFT_HIP(hipStreamSynchronize(s));
-    M->nsplit = nsplit;
+    *nsplit_out = nsplit;
     return SWIFT_OK;
+}
+
+// The primary segment table (used by every row-view kernel unless the narrow
+// row table applies).
+int build_segments(swift_matrix *M, int W, hipStream_t s) {
+    M->mp_items = W;
+    return build_seg_table(M, W, s, &M->seg, &M->nseg, &M->split_rows, &M->nsplit);
+}
+
+int build_alt_segments(swift_matrix *M, int W, hipStream_t s) {
+    M->alt_items = W;
+    return build_seg_table(M, W, s, &M->alt_seg, &M->alt_nseg, &M->alt_split_rows,
+                           &M->alt_nsplit);
 }
 
 // Row view of the regular part: stable radix sort of the regular entries by
@@ -934,7 +950,8 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
         M->nempty = ne;
     }
     SWIFT_HIP(hipStreamSynchronize(s));
-    return build_segments(M, 128, s);
+    SWIFT_TRY(build_segments(M, 128, s));
+    return build_alt_segments(M, 256, s);
 }
 
 int row_sort_impl(swift_matrix *M, hipStream_t s) {

@@ -396,8 +396,12 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.rrow = M->reside_cscrowidx;
     a.rval = rval;
     a.m = M->m;
-    a.nseg = M->nseg;
-    a.seg = M->seg;
+    // narrow B rows take the longer segments unless the size was set
+    const bool alt = !M->user_items && M->alt_seg && (size_t)K * sizeof(TB) <= 256;
+    a.nseg = alt ? M->alt_nseg : M->nseg;
+    a.seg = alt ? M->alt_seg : M->seg;
+    const int nsplit = alt ? M->alt_nsplit : M->nsplit;
+    const int *split_rows = alt ? M->alt_split_rows : M->split_rows;
     a.rv_ptr = M->rv_ptr;
     a.rv_col = M->rv_col;
     a.rv_val = rv_val;
@@ -419,9 +423,9 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     const bool flat = kernel == SWIFT_KERNEL_FLAT;
     const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW || flat;
     if (view) {
-        if (!a.accumulate && M->nsplit) {
-            hipLaunchKernelGGL(k_zero_rows<TC>, dim3(ceil_div((size_t)M->nsplit * 64, 256)),
-                               dim3(256), 0, s, M->split_rows, M->nsplit, (TC *)C, K, K);
+        if (!a.accumulate && nsplit) {
+            hipLaunchKernelGGL(k_zero_rows<TC>, dim3(ceil_div((size_t)nsplit * 64, 256)),
+                               dim3(256), 0, s, split_rows, nsplit, (TC *)C, K, K);
             SWIFT_CHECK_LAUNCH();
         }
         if (flat && !a.accumulate && M->nempty) {

@@ -163,6 +163,12 @@ struct swift_matrix {
     int mp_items = 0, nseg = 0, nsplit = 0;
     int2 *seg = nullptr;         // nseg+1 (row, entry) segment starts
     int *split_rows = nullptr;   // rows whose entries span a segment boundary
+    // second table for narrow B rows (K * element size <= 256 bytes), used
+    // unless the segment size was set explicitly (user_items != 0)
+    int alt_items = 0, alt_nseg = 0, alt_nsplit = 0;
+    int2 *alt_seg = nullptr;
+    int *alt_split_rows = nullptr;
+    int user_items = 0;
 
     mutable swift::Buffer scratch;  // reusable device workspace
 };
