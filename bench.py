@@ -236,6 +236,18 @@ def extras(S, torch, M, d, B, K):
     Ct = torch.empty((K, m), device="cuda", dtype=torch.float32)
     res["f32_col_ms"] = round(time_call(torch, lambda: M.spmm(Bt, out=Ct, layout=S.COL_MAJOR),
                                         reps=3, warm=1), 4)
+    # host buffers handed over (pinned): B in over PCIe, SpMM, C back out
+    Bhost = B.cpu().pin_memory()
+    Chost = torch.empty((m, K), dtype=torch.float32).pin_memory()
+    Bdev = torch.empty_like(B)
+
+    def pcie():
+        Bdev.copy_(Bhost, non_blocking=True)
+        M.spmm(Bdev, out=C)
+        Chost.copy_(C, non_blocking=True)
+
+    res["f32_row_pcie_inclusive_ms"] = round(time_call(torch, pcie, reps=5, warm=1), 4)
+    del Bhost, Chost, Bdev
     M.prepare_values(S.BF16)
     Bh = S.convert_dense(B, S.BF16)
     bf = time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16))
@@ -244,6 +256,7 @@ def extras(S, torch, M, d, B, K):
     dC = torch.randn((m, K), device="cuda")
     bw = time_call(torch, lambda: M.backward_fused(dC, B), reps=5)
     res["fwd_bwd_over_fwd"] = round((base + bw) / base, 3)
+    res["bwd_fused_ms"] = round(bw, 4)
     res["f32_row_gflops"] = round(flops / base / 1e6, 1)
     return res
 

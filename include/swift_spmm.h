@@ -171,9 +171,11 @@ void swift_destroy(swift_matrix *mat);
 int swift_row_idx_sort(swift_matrix *mat, swift_stream_t stream);
 
 /* Schedule knob of the row-view kernels: items (rows + entries) per
- * merge-path segment, i.e. per wave (default 128).  Rebuilds the segment
- * table on the GPU; the format itself is unchanged.  The batch autotuner
- * (swift_batch_create) sets it per matrix. */
+ * merge-path segment, i.e. per wave.  By default rows of B up to 256 bytes
+ * use 256 items and wider rows 128; a call here pins one size for every
+ * product of the matrix.  Rebuilds the segment table on the GPU; the format
+ * itself is unchanged.  The batch autotuner (swift_batch_create) and
+ * swift_autotune set it per matrix.  No counterpart in the reference. */
 int swift_set_segment_items(swift_matrix *mat, int items, swift_stream_t stream);
 
 /* Sizes + device pointers of the format (valid until swift_destroy). */
