@@ -73,32 +73,6 @@ __device__ __forceinline__ void load_b(const TB *__restrict__ p, TC (&o)[VEC]) {
     }
 }
 
-#ifndef SWIFT_RV_UNROLL
-#define SWIFT_RV_UNROLL 4
-#endif
-#ifndef SWIFT_RV_NT
-#define SWIFT_RV_NT 0
-#endif
-
-// Streaming (non-temporal) form of load_b for 16-byte FP32 and 8-byte 16-bit rows.
-template <class TB, int VEC, class TC>
-__device__ __forceinline__ void load_b_nt(const TB *__restrict__ p, TC (&o)[VEC]) {
-    if constexpr (std::is_same<TB, float>::value && VEC == 4) {
-        typedef float f4 __attribute__((ext_vector_type(4)));
-        f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(p));
-        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-    } else if constexpr (std::is_same<TB, __hip_bfloat16>::value && VEC == 4) {
-        typedef unsigned u2 __attribute__((ext_vector_type(2)));
-        u2 v = __builtin_nontemporal_load(reinterpret_cast<const u2 *>(p));
-        o[0] = __uint_as_float(v.x << 16);
-        o[1] = __uint_as_float(v.x & 0xFFFF0000u);
-        o[2] = __uint_as_float(v.y << 16);
-        o[3] = __uint_as_float(v.y & 0xFFFF0000u);
-    } else {
-        load_b<TB, VEC, TC>(p, o);
-    }
-}
-
 // Buffer-resource loads of B rows.  A raw buffer load whose byte offset is at
 // or past the resource's range returns zeros without touching memory, so an
 // unused slot takes the offset OOB_OFFSET instead of a branch around its load,
@@ -336,10 +310,18 @@ __device__ __forceinline__ double bperm_val(int addr, double x) {
 // row end the sub-groups' sums are combined with a butterfly and sub-group 0
 // stores the row.  Control flow is uniform across the wave.  Rows cut by a
 // segment boundary are split rows (zeroed beforehand) and take atomics.
+#ifndef SWIFT_UNROLL_NARROW
+#define SWIFT_UNROLL_NARROW 4
+#endif
+#ifndef SWIFT_UNROLL_WIDE
+#define SWIFT_UNROLL_WIDE 4
+#endif
+
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF>
 __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
     constexpr int EPS = 64 / LPE;
-    constexpr int UNROLL = SWIFT_RV_UNROLL;
+    // B-row loads in flight per lane group and step
+    constexpr int UNROLL = sizeof(TB) == 2 ? SWIFT_UNROLL_NARROW : SWIFT_UNROLL_WIDE;
     constexpr int STEP = EPS * UNROLL;  // entries per iteration
     const int lane = threadIdx.x & 63;
     const int h = lane / LPE, sub = lane % LPE;
@@ -372,7 +354,8 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
             rbase = row;
             rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
         }
-        const int rend = __shfl(rend_l, row - rbase, 64);
+        // wave-uniform: kept in scalar registers so the slot tests below branch
+        const int rend = __builtin_amdgcn_readfirstlane(__shfl(rend_l, row - rbase, 64));
         const int row_end = row < r1 ? rend : e1;
         while (e < row_end) {
             if (e - ebase >= 64) {
@@ -396,6 +379,9 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
                     const int base = (off + j + h) << 2;
 #pragma unroll
                     for (int u = 0; u < UNROLL; u++) {
+                        // a slot wholly past the row's end issues nothing (a
+                        // wave-uniform branch); a partly used one reads zeros
+                        if (u > 0 && j + u * EPS >= nb) break;
                         const int idx = j + u * EPS + h;
                         const int c = __builtin_amdgcn_ds_bpermute(base + ((u * EPS) << 2), mc);
                         const TC vs = bperm_val(base + ((u * EPS) << 2), mv);
@@ -412,11 +398,13 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
                         }
                     }
 #pragma unroll
-                    for (int u = 0; u < UNROLL; u++)
+                    for (int u = 0; u < UNROLL; u++) {
+                        if (u > 0 && j + u * EPS >= nb) break;
 #pragma unroll
                         for (int q = 0; q < Q; q++)
 #pragma unroll
                             for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * b[u][q][v];
+                    }
                     continue;
                 }
 #pragma unroll
@@ -431,10 +419,8 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
                         const int k0 = q * LPE * VEC + sub * VEC;
-                        if (idx < nb && (!MASK || k0 < a.kcount)) {
-                            if (SWIFT_RV_NT) load_b_nt<TB, VEC, TC>(brow + k0, b[u][q]);
-                            else load_b<TB, VEC, TC>(brow + k0, b[u][q]);
-                        }
+                        if (idx < nb && (!MASK || k0 < a.kcount))
+                            load_b<TB, VEC, TC>(brow + k0, b[u][q]);
                         else
 #pragma unroll
                             for (int v = 0; v < VEC; v++) b[u][q][v] = (TC)0;

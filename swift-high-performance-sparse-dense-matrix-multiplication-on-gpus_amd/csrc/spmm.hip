@@ -79,7 +79,8 @@ __global__ __launch_bounds__(256) void k_tiles_row(RowArgs a) {
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF>
 __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
     const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-    const int sid = (int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6);
+    // wave-uniform segment id, kept in a scalar register
+    const int sid = __builtin_amdgcn_readfirstlane((int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6));
     if (sid >= a.nseg) return;
     rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF>(a, sid);
 }
