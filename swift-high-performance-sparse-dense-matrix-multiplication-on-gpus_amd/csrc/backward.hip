@@ -215,6 +215,34 @@ __device__ __forceinline__ T group_sum(T v) {
     return v;
 }
 
+// N sums at once over the LPE lanes of a sub-group (N <= LPE, powers of two):
+// each halving step exchanges half of the values with the partner lane, so N
+// values cost N-1 + log2(LPE/N) shuffles instead of N*log2(LPE).  Lane sub
+// ends up with the full sum of value sub / (LPE/N).
+template <class T, int LPE, int N>
+__device__ __forceinline__ T multi_group_sum(const T (&d)[N], int sub) {
+    static_assert(N <= LPE && (N & (N - 1)) == 0, "N must be a power of two <= LPE");
+    T x[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = d[i];
+#pragma unroll
+    for (int lvl = 0; (N >> lvl) > 1; lvl++) {
+        const int off = LPE >> (lvl + 1);
+        const int half = (N >> lvl) / 2;
+        const bool up = (sub & off) != 0;
+#pragma unroll
+        for (int i = 0; i < half; i++) {
+            const T send = up ? x[i] : x[i + half];
+            const T keep = up ? x[i + half] : x[i];
+            x[i] = keep + __shfl_xor(send, off, 64);
+        }
+    }
+    T r = x[0];
+#pragma unroll
+    for (int off = (LPE / N) / 2; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+    return r;
+}
+
 template <class T, int VEC, int Q, int LPE, bool DB, bool DV, bool BUF>
 __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     constexpr int EPS = 64 / LPE;
@@ -223,7 +251,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     const int lane = threadIdx.x & 63;
     const int h = lane / LPE, sub = lane % LPE;
     const unsigned lb = xcd_block_bw(blockIdx.x, gridDim.x);
-    const int sid = (int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6);
+    const int sid = __builtin_amdgcn_readfirstlane((int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6));
     if (sid >= a.nseg) return;
     const int2 st = a.seg[sid], en = a.seg[sid + 1];
     const int r1 = en.x, e1 = en.y;
@@ -254,7 +282,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
             rbase = row;
             rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
         }
-        const int rend = __shfl(rend_l, row - rbase, 64);
+        const int rend = __builtin_amdgcn_readfirstlane(__shfl(rend_l, row - rbase, 64));
         const int row_end = row < r1 ? rend : e1;
         if (DV && e < row_end) {
 #pragma unroll
@@ -283,6 +311,8 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                     const int base = (off + j + h) << 2;
 #pragma unroll
                     for (int u = 0; u < UNROLL; u++) {
+                        // slots wholly past the row's end issue nothing
+                        if (u > 0 && j + u * EPS >= nb) break;
                         const int addr = base + ((u * EPS) << 2);
                         const int c = __builtin_amdgcn_ds_bpermute(addr, mc);
                         src[u] = __builtin_amdgcn_ds_bpermute(addr, ms);
@@ -317,23 +347,37 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                             for (int v = 0; v < VEC; v++) g[u][q][v] = (T)0;
                     }
                 }
+                if (DB) {
 #pragma unroll
-                for (int u = 0; u < UNROLL; u++) {
-                    if (DB) {
+                    for (int u = 0; u < UNROLL; u++) {
+                        if (BUF && u > 0 && j + u * EPS >= nb) break;  // not loaded
 #pragma unroll
                         for (int q = 0; q < Q; q++)
 #pragma unroll
                             for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * g[u][q][v];
                     }
-                    if (DV) {
-                        T d = (T)0;
+                }
+                if (DV) {
+                    // partial dots of the UNROLL slots, summed over the lane
+                    // group together; lane sub then holds slot sub / LEAD
+                    constexpr int LEAD = LPE / UNROLL;
+                    T d[UNROLL];
+                    int sv = 0;
+                    const int us = sub / LEAD;
 #pragma unroll
-                        for (int q = 0; q < Q; q++)
+                    for (int u = 0; u < UNROLL; u++) {
+                        T x = (T)0;
+                        if (!(BUF && u > 0 && j + u * EPS >= nb)) {
 #pragma unroll
-                            for (int v = 0; v < VEC; v++) d += g[u][q][v] * hr[q][v];
-                        d = group_sum<T, LPE>(d);
-                        if (sub == 0 && j + u * EPS + h < nb) static_cast<T *>(a.dvals)[src[u]] = d;
+                            for (int q = 0; q < Q; q++)
+#pragma unroll
+                                for (int v = 0; v < VEC; v++) x += g[u][q][v] * hr[q][v];
+                            if (us == u) sv = src[u];
+                        }
+                        d[u] = x;
                     }
+                    const T r = multi_group_sum<T, LPE, UNROLL>(d, sub);
+                    if (sub % LEAD == 0 && j + us * EPS + h < nb) static_cast<T *>(a.dvals)[sv] = r;
                 }
             }
             e += nb;
@@ -392,9 +436,14 @@ __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
     const T *G = static_cast<const T *>(a.G);
     const T *H = static_cast<const T *>(a.H);
     const T *rval = static_cast<const T *>(a.rval);
+    // lane-strided layout k = q*LPE*VEC + x*LPE + sub: for a fixed (q, x) the
+    // LPE lanes touch LPE consecutive elements, so each dB atomic instruction
+    // covers whole cache lines
     T gc[Q][VEC];
 #pragma unroll
-    for (int q = 0; q < Q; q++) ldv<T, VEC>(G + (size_t)c * a.K + q * LPE * VEC + sub * VEC, gc[q]);
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int x = 0; x < VEC; x++) gc[q][x] = G[(size_t)c * a.K + q * LPE * VEC + x * LPE + sub];
     for (int e0 = start; e0 < stop; e0 += EPS) {
         const int e = e0 + h;
         const bool ok = e < stop;
@@ -402,15 +451,10 @@ __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
         if (DV) {
             T d = (T)0;
 #pragma unroll
-            for (int q = 0; q < Q; q++) {
-                T hv[VEC];
-                if (ok) ldv<T, VEC>(H + (size_t)row * a.K + q * LPE * VEC + sub * VEC, hv);
-                else
+            for (int q = 0; q < Q; q++)
 #pragma unroll
-                    for (int v = 0; v < VEC; v++) hv[v] = (T)0;
-#pragma unroll
-                for (int v = 0; v < VEC; v++) d += hv[v] * gc[q][v];
-            }
+                for (int x = 0; x < VEC; x++)
+                    if (ok) d += H[(size_t)row * a.K + q * LPE * VEC + x * LPE + sub] * gc[q][x];
             d = group_sum<T, LPE>(d);
             if (ok && sub == 0) static_cast<T *>(a.dvals)[a.rsrc[e]] = d;
         }
@@ -421,7 +465,7 @@ __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
             for (int q = 0; q < Q; q++)
 #pragma unroll
                 for (int x = 0; x < VEC; x++)
-                    atomic_add_bw(drow + q * LPE * VEC + sub * VEC + x, v * gc[q][x]);
+                    atomic_add_bw(drow + q * LPE * VEC + x * LPE + sub, v * gc[q][x]);
         }
     }
 }
