@@ -190,12 +190,28 @@ __device__ __forceinline__ void ldv(const T *p, T (&o)[VEC]) {
     }
 }
 
+// The walked row's own operand row (H) is read once: non-temporal loads.
+template <class T, int VEC>
+__device__ __forceinline__ void ldv_stream(const T *p, T (&o)[VEC]) {
+    if constexpr (std::is_same<T, float>::value && VEC == 4) {
+        const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p));
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    } else if constexpr (std::is_same<T, double>::value && VEC == 2) {
+        const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p));
+        o[0] = v.x; o[1] = v.y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < VEC; i++) o[i] = ld_stream(p + i);
+    }
+}
+
+// dB rows are written once: non-temporal stores.
 template <class T, int VEC>
 __device__ __forceinline__ void stv(T *p, const T (&x)[VEC]) {
     if constexpr (std::is_same<T, float>::value && VEC == 4) {
-        *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
+        st_stream4(p, x[0], x[1], x[2], x[3]);
     } else if constexpr (std::is_same<T, double>::value && VEC == 2) {
-        *reinterpret_cast<double2 *>(p) = make_double2(x[0], x[1]);
+        st_stream2(p, x[0], x[1]);
     } else {
 #pragma unroll
         for (int i = 0; i < VEC; i++) p[i] = x[i];
@@ -264,13 +280,13 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     if constexpr (BUF) rsrc = make_rsrc(G, a.gbytes);
     // staged row ends and entries, as in the forward row-view kernel
     int rbase = row;
-    int rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+    int rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
     int ebase = e, mc = 0, ms = 0;
     T mv = (T)0;
     if (ebase + lane < e1) {
-        mc = a.rv_col[ebase + lane];
-        ms = a.rv_src[ebase + lane];
-        mv = val[ebase + lane];
+        mc = ld_stream(a.rv_col + ebase + lane);
+        ms = ld_stream(a.rv_src + ebase + lane);
+        mv = ld_stream(val + ebase + lane);
     }
     T acc[Q][VEC], hr[Q][VEC];
 #pragma unroll
@@ -280,13 +296,13 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     while (true) {
         if (row - rbase >= 64) {
             rbase = row;
-            rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+            rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
         }
         const int rend = __builtin_amdgcn_readfirstlane(__shfl(rend_l, row - rbase, 64));
         const int row_end = row < r1 ? rend : e1;
         if (DV && e < row_end) {
 #pragma unroll
-            for (int q = 0; q < Q; q++) ldv<T, VEC>(H + (size_t)row * a.K + q * LPE * VEC + sub * VEC, hr[q]);
+            for (int q = 0; q < Q; q++) ldv_stream<T, VEC>(H + (size_t)row * a.K + q * LPE * VEC + sub * VEC, hr[q]);
         }
         while (e < row_end) {
             if (e - ebase >= 64) {
@@ -294,9 +310,9 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 mc = ms = 0;
                 mv = (T)0;
                 if (ebase + lane < e1) {
-                    mc = a.rv_col[ebase + lane];
-                    ms = a.rv_src[ebase + lane];
-                    mv = val[ebase + lane];
+                    mc = ld_stream(a.rv_col + ebase + lane);
+                    ms = ld_stream(a.rv_src + ebase + lane);
+                    mv = ld_stream(val + ebase + lane);
                 }
             }
             const int off = e - ebase;

@@ -140,6 +140,35 @@ __device__ __forceinline__ void load_b_buf(__amdgpu_buffer_rsrc_t r, unsigned of
     }
 }
 
+// Streaming accesses.  The row view's (column, value, row end) arrays and the
+// finished output rows are touched once per launch; loading and storing them
+// with the non-temporal bit (nt) puts their lines first in line for eviction,
+// so L2 keeps the gathered B rows, which are reused.
+template <class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    if constexpr (sizeof(T) == 2) {
+        const unsigned short u =
+            __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+        T r;
+        __builtin_memcpy(&r, &u, 2);
+        return r;
+    } else {
+        return __builtin_nontemporal_load(p);
+    }
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void st_stream4(float *p, float x, float y, float z, float w) {
+    const f32x4 v = {x, y, z, w};
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(p));
+}
+__device__ __forceinline__ void st_stream2(double *p, double x, double y) {
+    const f64x2 v = {x, y};
+    __builtin_nontemporal_store(v, reinterpret_cast<f64x2 *>(p));
+}
+
 __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
 
@@ -227,16 +256,15 @@ __device__ __forceinline__ void load_strided(const RowArgs &a, int bcol, int sub
 
 // -------------------------------------------------------- row view kernel --
 
+// Output rows are written once: 16-byte non-temporal stores.
 template <class TC, int VEC>
 __device__ __forceinline__ void store_vec(TC *p, const TC (&x)[VEC]) {
     if constexpr (std::is_same<TC, float>::value && VEC % 4 == 0) {
 #pragma unroll
-        for (int i = 0; i < VEC; i += 4)
-            *reinterpret_cast<float4 *>(p + i) = make_float4(x[i], x[i + 1], x[i + 2], x[i + 3]);
+        for (int i = 0; i < VEC; i += 4) st_stream4(p + i, x[i], x[i + 1], x[i + 2], x[i + 3]);
     } else if constexpr (std::is_same<TC, double>::value && VEC % 2 == 0) {
 #pragma unroll
-        for (int i = 0; i < VEC; i += 2)
-            *reinterpret_cast<double2 *>(p + i) = make_double2(x[i], x[i + 1]);
+        for (int i = 0; i < VEC; i += 2) st_stream2(p + i, x[i], x[i + 1]);
     } else {
 #pragma unroll
         for (int i = 0; i < VEC; i++) p[i] = x[i];
@@ -335,14 +363,14 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
     if constexpr (BUF) rsrc = make_rsrc(B + a.kbase, a.bbytes);
     // staged row ends: lane l holds the end of row rbase + l
     int rbase = row;
-    int rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+    int rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
     // staged entries: lane l holds entry ebase + l
     int ebase = e, mc = 0;
     TC mv = (TC)0;
     if (ebase + lane < e1) {
-        const int c = a.rv_col[ebase + lane];
+        const int c = ld_stream(a.rv_col + ebase + lane);
         mc = a.inv_perm ? a.inv_perm[c] : c;
-        mv = to_acc(val[ebase + lane]);
+        mv = to_acc(ld_stream(val + ebase + lane));
     }
     TC acc[Q][VEC];
 #pragma unroll
@@ -352,7 +380,7 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
     while (true) {
         if (row - rbase >= 64) {
             rbase = row;
-            rend_l = (rbase + lane < a.m) ? a.rv_ptr[rbase + lane + 1] : e1;
+            rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
         }
         // wave-uniform: kept in scalar registers so the slot tests below branch
         const int rend = __builtin_amdgcn_readfirstlane(__shfl(rend_l, row - rbase, 64));
@@ -363,9 +391,9 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
                 mc = 0;
                 mv = (TC)0;
                 if (ebase + lane < e1) {
-                    const int c = a.rv_col[ebase + lane];
+                    const int c = ld_stream(a.rv_col + ebase + lane);
                     mc = a.inv_perm ? a.inv_perm[c] : c;
-                    mv = to_acc(val[ebase + lane]);
+                    mv = to_acc(ld_stream(val + ebase + lane));
                 }
             }
             const int off = e - ebase;
@@ -487,10 +515,10 @@ __device__ __forceinline__ void flat_walk(const RowArgs &a, int w) {
         r_ = -1;
         if (rel + sub < n) {
             const int e = e0 + rel + sub;
-            const int c = a.rv_col[e];
+            const int c = ld_stream(a.rv_col + e);
             c_ = a.inv_perm ? a.inv_perm[c] : c;
-            v_ = to_acc(val[e]);
-            r_ = a.rv_row[e];
+            v_ = to_acc(ld_stream(val + e));
+            r_ = ld_stream(a.rv_row + e);
         }
     };
     stage(0, mc, mv, mr);
