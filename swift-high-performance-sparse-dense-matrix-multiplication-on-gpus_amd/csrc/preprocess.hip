@@ -603,9 +603,14 @@ static void free_matrix_arrays(swift_matrix *M) {
                     M->rowsorted, M->rowsort_src, M->csr_rowptr, M->csr_colidx,
                     M->sortval_lp, M->reside_val_lp, M->work, M->rv_ptr, M->rv_col,
                     M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->seg, M->split_rows,
-                    M->rv_row, M->empty_rows, M->alt_seg, M->alt_split_rows};
+                    M->rv_row, M->empty_rows};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (auto &t : M->cls) {
+        if (t.seg) (void)hipFree(t.seg);
+        if (t.split_rows) (void)hipFree(t.split_rows);
+        t = swift_matrix::SegTable();
+    }
     M->scratch.release();
 }
 
@@ -885,10 +890,21 @@ int build_segments(swift_matrix *M, int W, hipStream_t s) {
     return build_seg_table(M, W, s, &M->seg, &M->nseg, &M->split_rows, &M->nsplit);
 }
 
-int build_alt_segments(swift_matrix *M, int W, hipStream_t s) {
-    M->alt_items = W;
-    return build_seg_table(M, W, s, &M->alt_seg, &M->alt_nseg, &M->alt_split_rows,
-                           &M->alt_nsplit);
+// The default tables.  A wave's segment of W items should keep enough B
+// bytes in flight without cutting many rows: W is the size measured best for
+// the class's B row bytes (64 items for >= 1 KiB rows up to 256 for narrow
+// rows), raised to 4 mean rows (items per row) so that long rows are rarely
+// split across segments.
+int build_class_segments(swift_matrix *M, hipStream_t s) {
+    static const int by_bytes[4] = {64, 96, 160, 256};
+    const long long m = std::max(M->m, 1);
+    const int mean = (int)((m + M->regular_nnz + m - 1) / m);
+    for (int c = 0; c < 4; c++) {
+        swift_matrix::SegTable &t = M->cls[c];
+        t.W = std::min(1024, std::max(by_bytes[c], 4 * mean));
+        SWIFT_TRY(build_seg_table(M, t.W, s, &t.seg, &t.nseg, &t.split_rows, &t.nsplit));
+    }
+    return SWIFT_OK;
 }
 
 // Row view of the regular part: stable radix sort of the regular entries by
@@ -951,7 +967,7 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
     }
     SWIFT_HIP(hipStreamSynchronize(s));
     SWIFT_TRY(build_segments(M, 128, s));
-    return build_alt_segments(M, 256, s);
+    return build_class_segments(M, s);
 }
 
 int row_sort_impl(swift_matrix *M, hipStream_t s) {

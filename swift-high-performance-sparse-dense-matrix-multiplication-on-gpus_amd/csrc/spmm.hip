@@ -397,12 +397,13 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.rrow = M->reside_cscrowidx;
     a.rval = rval;
     a.m = M->m;
-    // narrow B rows take the longer segments unless the size was set
-    const bool alt = !M->user_items && M->alt_seg && (size_t)K * sizeof(TB) <= 256;
-    a.nseg = alt ? M->alt_nseg : M->nseg;
-    a.seg = alt ? M->alt_seg : M->seg;
-    const int nsplit = alt ? M->alt_nsplit : M->nsplit;
-    const int *split_rows = alt ? M->alt_split_rows : M->split_rows;
+    // the default table of the B row size, unless the size was set
+    const swift_matrix::SegTable &tb = M->cls[seg_class((size_t)K * sizeof(TB))];
+    const bool dflt = !M->user_items && tb.seg;
+    a.nseg = dflt ? tb.nseg : M->nseg;
+    a.seg = dflt ? tb.seg : M->seg;
+    const int nsplit = dflt ? tb.nsplit : M->nsplit;
+    const int *split_rows = dflt ? tb.split_rows : M->split_rows;
     a.rv_ptr = M->rv_ptr;
     a.rv_col = M->rv_col;
     a.rv_val = rv_val;

@@ -16,6 +16,9 @@ def cases():
         "tall_2000x64": matgen.random_csr(2000, 64, 0.05, 4),
         "wide_40x3000": matgen.random_csr(40, 3000, 0.02, 5),
         "powerlaw_3000x2500": matgen.powerlaw_csr(3000, 2500, 6, max_len=400),
+        # the benchmark matrix's structure at 1/250 scale: even rows banded,
+        # odd rows Zipf lengths with uniform columns
+        "banded_mixed_4000": matgen.banded_mixed_csr(4000, 4000, 21),
     }
     # uniform columns, N % 32 == 0: no irregular part at all
     m = n = 64

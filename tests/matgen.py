@@ -81,6 +81,29 @@ def equal_rows_csr(m, n, seed):
     return _csr_from_lists(m, n, rows, rng)
 
 
+def banded_mixed_csr(m, n, seed, bw=64, lo=8, hi=32, zipf=1.8, max_len=200, offset=0,
+                     mixed=True):
+    """Even rows banded around column i + offset (bandwidth bw, lo..hi
+    entries), odd rows Zipf lengths with uniform columns (mixed) or banded too:
+    the small form of the 10^6-row benchmark matrix."""
+    rng = np.random.default_rng(seed)
+    ks = np.arange(1, max_len + 1, dtype=np.float64)
+    p = ks ** -zipf
+    p /= p.sum()
+    rows = []
+    for i in range(m):
+        if i % 2 == 0 or not mixed:
+            c0, c1 = max(0, i + offset - bw), min(n, i + offset + bw + 1)
+            k = min(int(rng.integers(lo, hi + 1)), max(0, c1 - c0))
+            c = np.sort(rng.choice(np.arange(c0, c1), size=k, replace=False)) if k else \
+                np.zeros(0, np.int64)
+        else:
+            k = min(int(rng.choice(ks.astype(np.int64), p=p)), n)
+            c = np.sort(rng.choice(n, size=k, replace=False))
+        rows.append(c)
+    return _csr_from_lists(m, n, rows, rng, vals="normal")
+
+
 def csr_to_dense(m, n, rowptr, colidx, val):
     A = np.zeros((m, n))
     for i in range(m):
