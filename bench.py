@@ -52,6 +52,22 @@ def measured_traffic(path):
     return None
 
 
+def keep_rows(d, parity):
+    """The same matrix with only the rows i % 2 == parity kept (the others
+    empty): the banded half (0) or the Zipf half (1), for diagnostics."""
+    rp = d["rowptr"].astype(np.int64)
+    m = d["m"]
+    lens = np.where(np.arange(m) % 2 == parity, np.diff(rp), 0)
+    nrp = np.zeros(m + 1, np.int64)
+    np.cumsum(lens, out=nrp[1:])
+    row = np.repeat(np.arange(m), lens)
+    src = rp[row] + (np.arange(int(nrp[-1])) - nrp[:-1][row])
+    out = dict(d)
+    out.update(rowptr=nrp.astype(np.int32), colidx=d["colidx"][src], val=d["val"][src],
+               nnz=int(nrp[-1]))
+    return out
+
+
 def cpu_baseline(d, K, seed):
     """Reference Swift_cpu (src/Swift_cpu.h:31-65), 1 thread, on the same matrix."""
     import oracle as O
@@ -91,6 +107,8 @@ def main():
                     help="also time the batched form over this many generated matrices")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 256 MiB device copy (counter calibration)")
+    ap.add_argument("--part", choices=("all", "banded", "zipf"), default="all",
+                    help="diagnostics: keep only the banded (even) or the Zipf (odd) rows")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the Swift SpMM path runs on a single MI355X "
@@ -112,6 +130,8 @@ def main():
     K, seed = args.K, 0
     t0 = time.time()
     d = S.generate(2, scale=args.scale, seed=seed)
+    if args.part != "all":
+        d = keep_rows(d, 0 if args.part == "banded" else 1)
     m, n, nnz = d["m"], d["n"], d["nnz"]
     log("matrix %dx%d nnz=%d generated in %.1fs" % (m, n, nnz, time.time() - t0))
     vals32 = d["val"].astype(np.float32)
@@ -157,7 +177,7 @@ def main():
     alg = algorithmic_bytes(m, n, nnz, K)
     achieved = alg / (reg_ms * 1e-3)
     # the committed counter summary covers the default workload only
-    default = args.scale == 1.0 and K == 128
+    default = args.scale == 1.0 and K == 128 and args.part == "all"
     traffic = measured_traffic(os.path.join(
         ROOT, "profiles", "rowview_f32_k128_1Mx1M_16Mnnz_traffic.txt")) if default else None
     out = {
@@ -177,6 +197,7 @@ def main():
                                "8..32 nnz, odd rows Zipf(1.8) <=20k), FP32, K=%d, row-major B "
                                "U[-1,1] in original column order" % (m, n, nnz, K),
                    "rows": m, "cols": n, "nnz": nnz, "K": K, "layout": "row-major",
+                   "rows_kept": args.part,
                    "build_ms": round(build_ms, 1),
                    "phase_ms": {"zero_C": round(zero_ms, 4), "regular": round(reg_ms, 4),
                                 "reside": round(res_ms, 4)}},

@@ -891,15 +891,16 @@ int build_segments(swift_matrix *M, int W, hipStream_t s) {
 }
 
 // The default tables.  A wave's segment of W items should keep enough B
-// bytes in flight without cutting many rows: W is the size measured best for
-// the class's B row bytes (64 items for >= 1 KiB rows up to 256 for narrow
-// rows), raised to 4 mean rows (items per row) so that long rows are rarely
-// split across segments.
+// bytes in flight without cutting many rows: W is the size measured best on
+// the 1M-row benchmark matrix for the class's B rows (64 items for >= 1 KiB
+// rows, 96 for >= 512 B, 192 / 128 for 256..511 B of 16-bit / wider
+// elements, 256 for narrower rows), raised to 4 mean rows (items per row) so
+// that long rows are rarely split across segments.
 int build_class_segments(swift_matrix *M, hipStream_t s) {
-    static const int by_bytes[4] = {64, 96, 160, 256};
+    static const int by_bytes[SEG_CLASSES] = {64, 96, 192, 128, 256};
     const long long m = std::max(M->m, 1);
     const int mean = (int)((m + M->regular_nnz + m - 1) / m);
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < SEG_CLASSES; c++) {
         swift_matrix::SegTable &t = M->cls[c];
         t.W = std::min(1024, std::max(by_bytes[c], 4 * mean));
         SWIFT_TRY(build_seg_table(M, t.W, s, &t.seg, &t.nseg, &t.split_rows, &t.nsplit));

@@ -398,7 +398,7 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.rval = rval;
     a.m = M->m;
     // the default table of the B row size, unless the size was set
-    const swift_matrix::SegTable &tb = M->cls[seg_class((size_t)K * sizeof(TB))];
+    const swift_matrix::SegTable &tb = M->cls[seg_class((size_t)K * sizeof(TB), sizeof(TB))];
     const bool dflt = !M->user_items && tb.seg;
     a.nseg = dflt ? tb.nseg : M->nseg;
     a.seg = dflt ? tb.seg : M->seg;

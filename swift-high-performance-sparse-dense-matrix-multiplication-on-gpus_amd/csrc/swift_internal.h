@@ -50,9 +50,14 @@ inline size_t value_size(swift_dtype t) {
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
-// Segment-table class of a product whose B rows are row_bytes long.
-inline int seg_class(size_t row_bytes) {
-    return row_bytes >= 1024 ? 0 : row_bytes >= 512 ? 1 : row_bytes >= 256 ? 2 : 3;
+// Segment-table class of a product whose B rows are row_bytes long, of
+// elements elem bytes wide.
+constexpr int SEG_CLASSES = 5;
+inline int seg_class(size_t row_bytes, size_t elem) {
+    if (row_bytes >= 1024) return 0;
+    if (row_bytes >= 512) return 1;
+    if (row_bytes >= 256) return elem == 2 ? 2 : 3;
+    return 4;
 }
 
 // Device workspace that grows on demand; owned by a matrix or a call site.
@@ -169,14 +174,14 @@ struct swift_matrix {
     int2 *seg = nullptr;         // nseg+1 (row, entry) segment starts
     int *split_rows = nullptr;   // rows whose entries span a segment boundary
     // Default tables by the bytes of one B row (K * element size) gathered
-    // per entry -- classes >= 1024, >= 512, >= 256 and < 256 bytes -- used
-    // unless the segment size was set explicitly (user_items != 0).
+    // per entry (swift::seg_class) -- used unless the segment size was set
+    // explicitly (user_items != 0).
     struct SegTable {
         int W = 0, nseg = 0, nsplit = 0;
         int2 *seg = nullptr;
         int *split_rows = nullptr;
     };
-    SegTable cls[4];
+    SegTable cls[swift::SEG_CLASSES];
     int user_items = 0;
 
     mutable swift::Buffer scratch;  // reusable device workspace
