@@ -87,6 +87,12 @@ typedef struct {
     const int *blkstart, *blkstop, *blkptr;
     const int *val_position, *val_offset, *rowsorted;
     const int *sortsrc, *reside_src; /* CSR position of each entry (ours) */
+    /* Build phase times in ms (host clock at the phase ends, where the build
+     * synchronises; ours, for the preprocessing-cost report of
+     * test/preprocess/main.cu:440-456): [0] column sort (csr->csc + ColSort),
+     * [1] blocking (staircase tiles, reside part, block_catgory, balance),
+     * [2] kernel schedules (work list, row view, segment tables), [3] total. */
+    float build_ms[4];
 } swift_info;
 
 /* ------------------------------------------------------------------ host */
@@ -182,6 +188,25 @@ int swift_set_segment_items(swift_matrix *mat, int items, swift_stream_t stream)
 
 /* Sizes + device pointers of the format (valid until swift_destroy). */
 int swift_get_info(const swift_matrix *mat, swift_info *info);
+
+/* Matrix statistics of the reference's study drivers, computed on the GPU
+ * from the built matrix: sparsity and the share of empty 32x32 blocks
+ * (test/MTX_info/main.cu:143-176, :484), the sizes of Swift's regular (tile)
+ * and irregular (reside) parts (test/irr_regular_ratio/main.cu:196-218, which
+ * counts tiles of its own variant of the tiling; here nonzeros of the two
+ * parts), the tile formats of block_catgory and the row/column extremes. */
+typedef struct {
+    double sparsity;            /* nnz / (m * n)                                */
+    double empty_block_ratio;   /* 32x32 blocks without an entry / all blocks   */
+    long long blocks, nonempty_blocks;
+    int regular_nnz, reside_nnz;
+    double regular_ratio;       /* regular_nnz / nnz                            */
+    int tilenum;
+    int tile_format_count[3];   /* tiles of format 0, 1, 2                      */
+    int max_row_nnz, max_col_nnz;
+    double mean_row_nnz;
+} swift_stats;
+int swift_matrix_stats(const swift_matrix *mat, swift_stats *stats, swift_stream_t stream);
 
 /* Permute dense rows into sorted column order: out[i,:] = in[perm[i],:]
  * (ROW) or out[:,i] = in[:,perm[i]] (COL).  Pre4sortedx

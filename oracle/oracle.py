@@ -240,6 +240,17 @@ def sddmm_f64(m, n, rowptr, colidx, dC, B):
     return out
 
 
+def block_stats(m, n, rowptr, colidx):
+    """(blocks, empty 32x32 blocks, sparsity nnz/(m*n)) of a CSR pattern
+    (test/MTX_info/main.cu:143-176, :484)."""
+    blocks, empty = C.c_longlong(), C.c_longlong()
+    lib().oracle_block_stats(m, n, _ip(np.ascontiguousarray(rowptr, np.int32)),
+                             _ip(np.ascontiguousarray(colidx, np.int32)),
+                             C.byref(blocks), C.byref(empty))
+    nnz = int(rowptr[-1]) if len(rowptr) else 0
+    return blocks.value, empty.value, (nnz / (m * n) if m and n else 0.0)
+
+
 # ------------------------------------------------------------------ _ref --
 
 class _RefView(C.Structure):

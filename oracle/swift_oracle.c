@@ -585,3 +585,22 @@ void oracle_sddmm_csr_f64(int m, int n, const int *rowptr, const int *colidx,
             dvals[e] = acc;
         }
 }
+
+/* test/MTX_info/main.cu:143-176: an entry count per 32x32 block (row-major
+ * over block rows), then the blocks whose count is zero. */
+void oracle_block_stats(int m, int n, const int *rowptr, const int *colidx,
+                        long long *blocks, long long *empty) {
+    const long long rows = m % 32 == 0 ? m / 32 : m / 32 + 1;
+    const long long cols = n % 32 == 0 ? n / 32 : n / 32 + 1;
+    const long long total = rows * cols;
+    int *count = (int *)calloc(total > 0 ? (size_t)total : 1, sizeof(int));
+    for (int i = 0; i < m; i++)
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++)
+            count[(long long)(i / 32) * cols + colidx[j] / 32]++;
+    long long e = 0;
+    for (long long b = 0; b < total; b++)
+        if (count[b] == 0) e++;
+    free(count);
+    *blocks = total;
+    *empty = e;
+}

@@ -112,6 +112,11 @@ void oracle_sddmm_csr_f64(int m, int n, const int *rowptr, const int *colidx,
                           int K, const double *dC, const double *B,
                           double *dvals);
 
+/* Empty 32x32 blocks of the CSR pattern, test/MTX_info/main.cu:143-176:
+ * *blocks = ceil(m/32)*ceil(n/32), *empty = blocks holding no entry. */
+void oracle_block_stats(int m, int n, const int *rowptr, const int *colidx,
+                        long long *blocks, long long *empty);
+
 #ifdef __cplusplus
 }
 #endif

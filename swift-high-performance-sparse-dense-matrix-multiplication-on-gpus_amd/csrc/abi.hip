@@ -40,6 +40,7 @@ int backward_impl(const swift_matrix *AT, int K, swift_dtype dtype, const void *
 int sddmm_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, const void *B,
                void *dvals, hipStream_t s);
 int update_values_impl(swift_matrix *M, const void *csr, hipStream_t s);
+int matrix_stats_impl(const swift_matrix *M, swift_stats *o, hipStream_t s);
 }  // namespace swift
 
 using namespace swift;
@@ -181,6 +182,15 @@ extern "C" int swift_set_segment_items(swift_matrix *mat, int items, swift_strea
                 return rc;)
 }
 
+extern "C" int swift_matrix_stats(const swift_matrix *mat, swift_stats *stats,
+                                  swift_stream_t stream) {
+    if (!mat || !stats) {
+        set_error("swift_matrix_stats: bad arguments");
+        return SWIFT_ERR_ARG;
+    }
+    SWIFT_GUARD(return matrix_stats_impl(mat, stats, S(stream));)
+}
+
 extern "C" int swift_get_info(const swift_matrix *M, swift_info *o) {
     if (!M || !o) {
         set_error("swift_get_info: bad arguments");
@@ -205,6 +215,7 @@ extern "C" int swift_get_info(const swift_matrix *M, swift_info *o) {
     o->val_position = M->val_position; o->val_offset = M->val_offset;
     o->rowsorted = M->rowsorted;
     o->sortsrc = M->sortsrc; o->reside_src = M->reside_src;
+    for (int i = 0; i < 4; i++) o->build_ms[i] = M->build_ms[i];
     return SWIFT_OK;
 }
 

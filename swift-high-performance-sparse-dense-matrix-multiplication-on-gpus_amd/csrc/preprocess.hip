@@ -10,6 +10,7 @@
 #include "swift_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 namespace swift {
@@ -670,10 +671,28 @@ int build_work_list(swift_matrix *M, int depth, hipStream_t s) {
     return SWIFT_OK;
 }
 
+// Host clock in ms; the build synchronises its stream at each phase end, so
+// phase times taken on the host are the phases' device times plus launch and
+// allocation overhead, the way the reference times its preprocessing
+// (gettimeofday, test/preprocess/main.cu:440-456).
+static double wall_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
 int build_impl(swift_matrix *M, const int *d_rowptr, const int *d_colidx,
                const void *d_val, hipStream_t s) {
     const int m = M->m, n = M->n, nnz = M->nnz;
     const int vs = vsize_of(M->vtype);
+    const double t0 = wall_ms();
+    double t_sort = t0, t_block = t0;
+    auto stamp = [&]() {
+        const double t = wall_ms();
+        M->build_ms[0] = (float)(t_sort - t0);
+        M->build_ms[1] = (float)(t_block - t_sort);
+        M->build_ms[2] = (float)(t - t_block);
+        M->build_ms[3] = (float)(t - t0);
+    };
     // keep the input pattern (transposed layout, SDDMM, value updates)
     SWIFT_TRY(dalloc(&M->csr_rowptr, (size_t)m + 1));
     SWIFT_TRY(dalloc(&M->csr_colidx, (size_t)nnz));
@@ -715,7 +734,12 @@ int build_impl(swift_matrix *M, const int *d_rowptr, const int *d_colidx,
                                 M->perm, M->cptr, &M->max_col_nnz, T, s));
         SWIFT_HIP(hipStreamSynchronize(s));
     }
-    if (n == 0) return build_rowview(M, s);
+    t_sort = t_block = wall_ms();
+    if (n == 0) {
+        SWIFT_TRY(build_rowview(M, s));
+        stamp();
+        return SWIFT_OK;
+    }
     // 3. steps
     Buffer flag, idx, ntile, sw, tmp, small;
     SWIFT_TRY(flag.alloc(sizeof(int) * ((size_t)n + 1)));
@@ -837,10 +861,12 @@ int build_impl(swift_matrix *M, const int *d_rowptr, const int *d_colidx,
     SWIFT_HIP(hipMemcpyAsync(&numblk, small.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     SWIFT_HIP(hipStreamSynchronize(s));
     M->numblk = numblk;
+    t_block = wall_ms();
     // 8. kernel schedules: column-tile work list and the row view
     SWIFT_TRY(build_work_list(M, 16, s));
     SWIFT_TRY(build_rowview(M, s));
     SWIFT_HIP(hipStreamSynchronize(s));
+    stamp();
     return SWIFT_OK;
 }
 

@@ -184,5 +184,10 @@ struct swift_matrix {
     SegTable cls[swift::SEG_CLASSES];
     int user_items = 0;
 
+    // build phase times (ms): column sort (csr->csc + ColSort), blocking
+    // (staircase tiles, reside part, categorisation, balance), kernel
+    // schedules (work list, row view, segment tables), total
+    float build_ms[4] = {0, 0, 0, 0};
+
     mutable swift::Buffer scratch;  // reusable device workspace
 };

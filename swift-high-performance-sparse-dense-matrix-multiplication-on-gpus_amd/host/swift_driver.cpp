@@ -13,20 +13,43 @@
 // minimum time over passing variants, appended as one line in the
 // reference's result format (src/main.cu:460).
 //
-// Extra options: --fp32 (default is double, like src/), --seed N (default
-// time(0) as in the reference), --reps N, --out FILE, --gen W [--scale S] to
-// use a seeded configuration matrix instead of a file.
+// --mode selects one of the reference's study drivers instead; each appends
+// its result line in that driver's format, so the reference's DataProcess
+// parsers (which read fixed token positions) read ours:
+//   main        the flow above (default)
+//   ksweep      K sweep of test/irregular_K (script/Run_irregular_K.sh: K =
+//               24 48 96 192 384 768, or --ks), one file per K:
+//               "<name> m M n N width K nnz NNZ rocSPARSE T final T"
+//   cma         row- vs column-major dense operand (test/k-32/CMA-compare):
+//               "... withoutCMA T withCMA T"
+//   irropt      irregular part per reside column vs per balance block
+//               (test/k-32/irr_opt): "... non-opt T opt T"
+//   preprocess  preprocessing cost (test/preprocess/main.cu:440-456):
+//               "... sort T ratio R block T ratio R preprocess T final T overall_time T"
+//   mtxinfo     matrix statistics (test/MTX_info/main.cu:484-493 and
+//               test/irr_regular_ratio/main.cu:216): "... rocSPARSE: T
+//               sparsity S emptyblkratio E final T" and "<name> nnz NNZ
+//               regulatpart R f irregularpart I f" (nonzeros of Swift's two
+//               parts here, tiles of its variant tiling there)
+//
+// Options: --fp32 (default is double, like src/), --seed N (default time(0)
+// as in the reference), --reps N, --out FILE (result file; ksweep appends
+// _K<k>.txt to it), --ks K1,K2,..., --gen W [--scale S] to use a seeded
+// configuration matrix instead of a file.
 #include <hip/hip_runtime.h>
 #include <rocsparse/rocsparse.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
 #include <string>
+#include <sys/stat.h>
 #include <sys/time.h>
+#include <type_traits>
 #include <vector>
 
 #include "swift_spmm.h"
@@ -158,85 +181,187 @@ struct Args {
     int gen = 0;
     double scale = 1.0;
     std::string out;
+    std::string mode = "main";
+    std::vector<int> ks = {24, 48, 96, 192, 384, 768};  // script/Run_irregular_K.sh
+};
+
+// One loaded problem: the pattern, random values (src/main.cu:125-137) and the
+// Swift format built on the GPU.
+template <class T>
+struct Problem {
+    std::string name;
+    int m = 0, n = 0, nnz = 0;
+    int *rp = nullptr, *ci = nullptr;
+    std::vector<T> val;
+    swift_matrix *M = nullptr;
+    swift_info inf{};
+    std::vector<int> perm;
+    double t_build = 0, t_rowsort = 0;
+    ~Problem() {
+        if (M) swift_destroy(M);
+        swift_host_free(rp);
+        swift_host_free(ci);
+    }
 };
 
 template <class T>
-static int run(const Args &a) {
-    int m, n, nnz, sym = 0;
-    int *rp = nullptr, *ci = nullptr;
+static int load(const Args &a, Problem<T> &P) {
+    int sym = 0;
     double *v64 = nullptr;
-    std::string name;
     if (a.gen) {
-        int rc = swift_generate(a.gen, 0, a.scale, 0, &m, &n, &nnz, &rp, &ci, &v64, nullptr);
+        int rc = swift_generate(a.gen, 0, a.scale, 0, &P.m, &P.n, &P.nnz, &P.rp, &P.ci, &v64,
+                                nullptr);
         if (rc) { fprintf(stderr, "swift_generate: %d\n", rc); return 1; }
-        name = "config" + std::to_string(a.gen);
+        P.name = "config" + std::to_string(a.gen);
     } else {
-        int rc = swift_mmio_allinone(&m, &n, &nnz, &sym, &rp, &ci, &v64, a.path.c_str());
+        int rc = swift_mmio_allinone(&P.m, &P.n, &P.nnz, &sym, &P.rp, &P.ci, &v64, a.path.c_str());
         if (rc) {
             fprintf(stderr, "mmio_allinone(%s) failed: %d\n", a.path.c_str(), rc);
             return 1;
         }
-        name = a.path.substr(a.path.find_last_of("/\\") + 1);
-        name = name.substr(0, name.find_last_of('.'));
+        P.name = a.path.substr(a.path.find_last_of("/\\") + 1);
+        P.name = P.name.substr(0, P.name.find_last_of('.'));
     }
-    const int K = a.K;
-    printf("read success,input matrix %s :(%i,%i) nnz =%i  width=%d  %s\n", name.c_str(), m, n,
-           nnz, K, Traits<T>::name());
-    // random dense matrix and values (src/main.cu:125-137)
+    swift_host_free(v64);
+    // random values (src/main.cu:125-137); the dense operands follow per K
     unsigned seed = a.seed >= 0 ? (unsigned)a.seed : (unsigned)time(0);
     srand(seed);
-    std::vector<T> B((size_t)n * K), val(nnz);
-    for (auto &x : B) x = T(rand() % 100 - 50) / 100;
-    for (auto &x : val) x = T(rand() % 1000 - 500) / 1000;
-    swift_host_free(v64);
-    // golden (src/main.cu:145-155), value type T, row-major
-    std::vector<T> golden((size_t)m * K, T(0));
-    for (int i = 0; i < m; i++)
-        for (int e = rp[i]; e < rp[i + 1]; e++)
-            for (int k = 0; k < K; k++)
-                golden[(size_t)i * K + k] += val[e] * B[(size_t)ci[e] * K + k];
-    // vendor SpMM, column-major
-    std::vector<T> Bcol((size_t)n * K), Ccol((size_t)m * K), Gcol((size_t)m * K);
-    to_col_major(n, K, B.data(), Bcol.data());
-    to_col_major(m, K, golden.data(), Gcol.data());
-    float t_vendor = vendor_spmm<T>(m, n, nnz, K, rp, ci, val.data(), Bcol.data(), Ccol.data(),
-                                    a.reps);
-    int bad = verify(Ccol.data(), Gcol.data(), Ccol.size());
-    if (bad) {
-        printf("error rocSPARSE SpMM, error = %d\n", bad);
-        t_vendor = -1111;
-    } else {
-        printf("success rocSPARSE SpMM, process:%f ms\n", t_vendor);
-    }
+    P.val.resize(P.nnz);
+    for (auto &x : P.val) x = T(rand() % 1000 - 500) / 1000;
+    printf("read success,input matrix %s :(%i,%i) nnz =%i  %s\n", P.name.c_str(), P.m, P.n,
+           P.nnz, Traits<T>::name());
     // Swift preprocessing on the GPU
     double t0 = now_ms();
-    swift_matrix *M = nullptr;
-    int rc = swift_build_host(&M, m, n, nnz, Traits<T>::st, rp, ci, val.data());
-    double t_build = now_ms() - t0;
+    int rc = swift_build_host(&P.M, P.m, P.n, P.nnz, Traits<T>::st, P.rp, P.ci, P.val.data());
+    P.t_build = now_ms() - t0;
     if (rc) {
         fprintf(stderr, "swift_build: %s\n", swift_last_error());
         return 1;
     }
     t0 = now_ms();
-    rc = swift_row_idx_sort(M, nullptr);
+    rc = swift_row_idx_sort(P.M, nullptr);
     HIP_OK(hipDeviceSynchronize());
-    double t_rowsort = now_ms() - t0;
+    P.t_rowsort = now_ms() - t0;
     if (rc) {
         fprintf(stderr, "swift_row_idx_sort: %s\n", swift_last_error());
         return 1;
     }
-    swift_info inf;
-    swift_get_info(M, &inf);
+    swift_get_info(P.M, &P.inf);
     printf("Swift format: tilenum %d regular_nnz %d reside_col %d reside_nnz %d numblk %d "
-           "(build %.2f ms, row sort %.2f ms)\n", inf.tilenum, inf.regular_nnz, inf.reside_col,
-           inf.reside_nnz, inf.numblk, t_build, t_rowsort);
-    // sorted dense matrix (Pre4sortedx) in both layouts
-    std::vector<int> perm(n);
-    if (n) HIP_OK(hipMemcpy(perm.data(), inf.perm, sizeof(int) * n, hipMemcpyDeviceToHost));
-    std::vector<T> sB((size_t)n * K), sBcol((size_t)n * K);
+           "(build %.2f ms: sort %.2f block %.2f schedules %.2f; row sort %.2f ms)\n",
+           P.inf.tilenum, P.inf.regular_nnz, P.inf.reside_col, P.inf.reside_nnz, P.inf.numblk,
+           P.t_build, P.inf.build_ms[0], P.inf.build_ms[1], P.inf.build_ms[2], P.t_rowsort);
+    P.perm.resize(P.n);
+    if (P.n) HIP_OK(hipMemcpy(P.perm.data(), P.inf.perm, sizeof(int) * P.n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Dense operands of one K: random B (src/main.cu:125-130), its column-sorted
+// copy (Pre4sortedx) in both layouts, and the naive CSR golden
+// (src/main.cu:145-155) in both layouts.
+template <class T>
+struct Operands {
+    int K = 0;
+    std::vector<T> B, Bcol, sB, sBcol, golden, Gcol;
+};
+
+template <class T>
+static void make_operands(const Problem<T> &P, int K, Operands<T> &o) {
+    const int m = P.m, n = P.n;
+    o.K = K;
+    o.B.resize((size_t)n * K);
+    for (auto &x : o.B) x = T(rand() % 100 - 50) / 100;
+    o.golden.assign((size_t)m * K, T(0));
+    for (int i = 0; i < m; i++)
+        for (int e = P.rp[i]; e < P.rp[i + 1]; e++)
+            for (int k = 0; k < K; k++)
+                o.golden[(size_t)i * K + k] += P.val[e] * o.B[(size_t)P.ci[e] * K + k];
+    o.Bcol.resize((size_t)n * K);
+    o.Gcol.resize((size_t)m * K);
+    to_col_major(n, K, o.B.data(), o.Bcol.data());
+    to_col_major(m, K, o.golden.data(), o.Gcol.data());
+    o.sB.resize((size_t)n * K);
+    o.sBcol.resize((size_t)n * K);
     for (int i = 0; i < n; i++)
-        std::copy(&B[(size_t)perm[i] * K], &B[(size_t)perm[i] * K] + K, &sB[(size_t)i * K]);
-    to_col_major(n, K, sB.data(), sBcol.data());
+        std::copy(&o.B[(size_t)P.perm[i] * K], &o.B[(size_t)P.perm[i] * K] + K,
+                  &o.sB[(size_t)i * K]);
+    to_col_major(n, K, o.sB.data(), o.sBcol.data());
+}
+
+// rocSPARSE (col-major) with ResultVerify; -1111 when it disagrees.
+template <class T>
+static float vendor_time(const Problem<T> &P, const Operands<T> &o, int reps) {
+    std::vector<T> Ccol((size_t)P.m * o.K);
+    float t = vendor_spmm<T>(P.m, P.n, P.nnz, o.K, P.rp, P.ci, P.val.data(), o.Bcol.data(),
+                             Ccol.data(), reps);
+    int bad = verify(Ccol.data(), o.Gcol.data(), Ccol.size());
+    if (bad) {
+        printf("error rocSPARSE SpMM, error = %d\n", bad);
+        return -1111;
+    }
+    printf("success rocSPARSE SpMM, process:%f ms\n", t);
+    return t;
+}
+
+// One Swift variant on host pointers (swift_spmm_host: kernel-only events,
+// median, ResultVerify); -1 when it fails the check.
+template <class T>
+static float swift_time(const Problem<T> &P, const Operands<T> &o, swift_layout layout,
+                        swift_kernel kernel, int reps, const char *label) {
+    const bool row = layout == SWIFT_ROW_MAJOR;
+    std::vector<T> C((size_t)P.m * o.K);
+    float ms = -1;
+    double gf = 0;
+    int rc = swift_spmm_host(P.M, o.K, layout, Traits<T>::st, kernel,
+                             row ? o.sB.data() : o.sBcol.data(), C.data(),
+                             row ? o.golden.data() : o.Gcol.data(), 2, reps, &ms, &gf);
+    if (rc == SWIFT_OK) {
+        printf("success %-44s %10.4f ms %9.2f GFLOP/s\n", label, ms, gf);
+        return ms;
+    }
+    printf("error %-46s %s\n", label, swift_last_error());
+    return -1;
+}
+
+static float min_passing(std::initializer_list<float> ts) {
+    float best = -1;
+    for (float t : ts)
+        if (t >= 0 && (best < 0 || t < best)) best = t;
+    return best;
+}
+
+// Appends one line to the result file, creating its directory (the
+// reference's drivers write into ./data/).
+static void append(const std::string &path, const char *fmt, ...) {
+    const size_t slash = path.find_last_of('/');
+    if (slash != std::string::npos && slash > 0) {
+        std::string dir = path.substr(0, slash);
+        for (size_t i = 1; i <= dir.size(); i++)
+            if (i == dir.size() || dir[i] == '/') mkdir(dir.substr(0, i).c_str(), 0755);
+    }
+    FILE *f = fopen(path.c_str(), "a");
+    if (!f) {
+        fprintf(stderr, "Writing results fails (%s).\n", path.c_str());
+        return;
+    }
+    va_list ap;
+    va_start(ap, fmt);
+    vfprintf(f, fmt, ap);
+    va_end(ap);
+    fclose(f);
+}
+
+template <class T>
+static std::string out_path(const Args &a, const std::string &stem) {
+    return a.out.empty() ? "./data/" + stem + "_" + Traits<T>::name() + ".txt" : a.out;
+}
+
+// main: every variant, minimum over the passing ones (src/main.cu:443-461).
+template <class T>
+static int mode_main(const Args &a, Problem<T> &P) {
+    Operands<T> o;
+    make_operands(P, a.K, o);
+    const float t_vendor = vendor_time(P, o, a.reps);
     struct Variant {
         const char *name;
         swift_layout layout;
@@ -249,40 +374,195 @@ static int run(const Args &a) {
         {"Swift_GPU_4 (col-major, row-sorted groups)", SWIFT_COL_MAJOR, SWIFT_KERNEL_ROWSORT},
         {"Swift row-major default", SWIFT_ROW_MAJOR, SWIFT_KERNEL_DEFAULT},
     };
-    std::vector<T> C((size_t)m * K);
     float best = -1;
-    for (auto &v : variants) {
-        bool row = v.layout == SWIFT_ROW_MAJOR;
-        float ms = -1;
-        double gf = 0;
-        rc = swift_spmm_host(M, K, v.layout, Traits<T>::st, v.kernel,
-                             row ? sB.data() : sBcol.data(), C.data(),
-                             row ? golden.data() : Gcol.data(), 2, a.reps, &ms, &gf);
-        if (rc == SWIFT_OK) {
-            printf("success %-44s %10.4f ms %9.2f GFLOP/s\n", v.name, ms, gf);
-            if (best < 0 || ms < best) best = ms;
-        } else {
-            printf("error %-46s %s\n", v.name, swift_last_error());
-        }
-    }
-    swift_destroy(M);
-    std::string out = a.out.empty() ? "./data/results_Swift_" + std::string(Traits<T>::name()) +
-                                          "_" + std::to_string(K) + ".txt"
-                                    : a.out;
-    FILE *f = fopen(out.c_str(), "a");
+    for (auto &v : variants) best = min_passing({best, swift_time(P, o, v.layout, v.kernel, a.reps, v.name)});
+    const std::string out = a.out.empty() ? "./data/results_Swift_" + std::string(Traits<T>::name()) +
+                                                "_" + std::to_string(a.K) + ".txt"
+                                          : a.out;
     if (best >= 0) {
         printf("Swift GPU SpMM time %f ms\n", best);
-        if (f) fprintf(f, "%s m %d n %d width %d nnz %d rocSPARSE: %f Swift %f \n", name.c_str(),
-                       m, n, K, nnz, t_vendor, best);
-    } else {
-        printf("Swift GPU SpMM Check NO PASS!\n");
-        if (f) fprintf(f, "erro Swift (dense mtx col-major)%s \n", name.c_str());
+        append(out, "%s m %d n %d width %d nnz %d rocSPARSE: %f Swift %f \n", P.name.c_str(), P.m,
+               P.n, a.K, P.nnz, t_vendor, best);
+        return 0;
     }
-    if (f) fclose(f);
-    else fprintf(stderr, "Writing results fails (%s).\n", out.c_str());
-    swift_host_free(rp);
-    swift_host_free(ci);
+    printf("Swift GPU SpMM Check NO PASS!\n");
+    append(out, "erro Swift (dense mtx col-major)%s \n", P.name.c_str());
+    return 2;
+}
+
+// ksweep: test/irregular_K, one result file per K (FP32 and FP64 label
+// formats of test/irregular_K/FP32/main.cu:482 and FP64/main.cu:467).
+template <class T>
+static int mode_ksweep(const Args &a, Problem<T> &P) {
+    int rc = 0;
+    const std::string base = a.out.empty() ? "./data/Swift_irregular_K_" + std::string(Traits<T>::name())
+                                           : a.out;
+    for (int K : a.ks) {
+        printf("---- K = %d\n", K);
+        Operands<T> o;
+        make_operands(P, K, o);
+        const float tv = vendor_time(P, o, a.reps);
+        const float tr = swift_time(P, o, SWIFT_ROW_MAJOR, SWIFT_KERNEL_DEFAULT, a.reps,
+                                    "Swift row-major default");
+        const float tc = swift_time(P, o, SWIFT_COL_MAJOR, SWIFT_KERNEL_DEFAULT, a.reps,
+                                    "Swift col-major default");
+        const float best = min_passing({tr, tc});
+        const std::string out = base + "_K" + std::to_string(K) + ".txt";
+        if (best >= 0) {
+            if (std::is_same<T, float>::value)
+                append(out, "%s m %d n %d width %d nnz %d rocSPARSE %f final %f \n", P.name.c_str(),
+                       P.m, P.n, K, P.nnz, tv, best);
+            else
+                append(out, "%s m %d n %d width %d nnz %d rocSPARSE: %f final %f \n",
+                       P.name.c_str(), P.m, P.n, K, P.nnz, tv, best);
+        } else {
+            append(out, "erro Swift (dense mtx col-major)%s \n", P.name.c_str());
+            rc = 2;
+        }
+    }
+    return rc;
+}
+
+// cma: the reference's Swift kernels with a row-major dense operand
+// (Swift_gpu2 order, "withoutCMA") against the column-major one (Swift_GPU,
+// "withCMA"), test/k-32/CMA-compare/main.cu:316-357; the tuned kernels of
+// both layouts are printed beside them.
+template <class T>
+static int mode_cma(const Args &a, Problem<T> &P) {
+    Operands<T> o;
+    make_operands(P, a.K, o);
+    const float t_row = swift_time(P, o, SWIFT_ROW_MAJOR, SWIFT_KERNEL_BALANCED, a.reps,
+                                   "Swift_gpu2 (row-major, balanced)");
+    const float t_col = swift_time(P, o, SWIFT_COL_MAJOR, SWIFT_KERNEL_BALANCED, a.reps,
+                                   "Swift_GPU (col-major, balanced)");
+    swift_time(P, o, SWIFT_ROW_MAJOR, SWIFT_KERNEL_DEFAULT, a.reps, "Swift row-major default");
+    swift_time(P, o, SWIFT_COL_MAJOR, SWIFT_KERNEL_DEFAULT, a.reps, "Swift col-major default");
+    append(out_path<T>(a, "Swift_CMA_compare"), "%s m %d n %d width %d nnz %d withoutCMA %f withCMA %f \n",
+           P.name.c_str(), P.m, P.n, a.K, P.nnz, t_row, t_col);
+    return (t_row >= 0 && t_col >= 0) ? 0 : 2;
+}
+
+// irropt: the irregular (reside) part per reside column (Swift_gpu1's
+// unbalanced launch) against per balance block (Swift_GPU), kernel time of
+// that part alone (swift_spmm_timed phase 2, median), each product checked
+// with ResultVerify; test/k-32/irr_opt/main.cu:300-372.
+template <class T>
+static int mode_irropt(const Args &a, Problem<T> &P) {
+    Operands<T> o;
+    make_operands(P, a.K, o);
+    const int K = a.K;
+    T *dB, *dC;
+    HIP_OK(hipMalloc(&dB, sizeof(T) * std::max<size_t>(1, (size_t)P.n * K)));
+    HIP_OK(hipMalloc(&dC, sizeof(T) * std::max<size_t>(1, (size_t)P.m * K)));
+    HIP_OK(hipMemcpy(dB, o.sBcol.data(), sizeof(T) * (size_t)P.n * K, hipMemcpyHostToDevice));
+    std::vector<T> C((size_t)P.m * K);
+    float t[2] = {-1, -1};
+    const swift_kernel ks[2] = {SWIFT_KERNEL_UNBALANCED, SWIFT_KERNEL_BALANCED};
+    for (int v = 0; v < 2; v++) {
+        std::vector<float> r;
+        int rc = SWIFT_OK;
+        for (int it = 0; it < a.reps + 2 && rc == SWIFT_OK; it++) {
+            float ms[3];
+            rc = swift_spmm_timed(P.M, K, SWIFT_COL_MAJOR, Traits<T>::st, ks[v], SWIFT_B_SORTED,
+                                  dB, dC, nullptr, ms);
+            if (it >= 2) r.push_back(ms[2]);
+        }
+        if (rc != SWIFT_OK) {
+            printf("error %s\n", swift_last_error());
+            continue;
+        }
+        HIP_OK(hipMemcpy(C.data(), dC, sizeof(T) * C.size(), hipMemcpyDeviceToHost));
+        const int bad = verify(C.data(), o.Gcol.data(), C.size());
+        std::sort(r.begin(), r.end());
+        t[v] = bad ? -1 : r[r.size() / 2];
+        printf("%s Swift GPU SpMM irregular part %s: %f ms\n", bad ? "error" : "success",
+               v ? "with optimization (balance blocks)" : "without optimization (per column)",
+               t[v]);
+    }
+    (void)hipFree(dB);
+    (void)hipFree(dC);
+    if (t[0] >= 0 && t[1] >= 0) {
+        append(out_path<T>(a, "Swift_irr_opt"), "%s m %d n %d width %d nnz %d non-opt %f opt %f\n",
+               P.name.c_str(), P.m, P.n, K, P.nnz, t[0], t[1]);
+        return 0;
+    }
+    append(out_path<T>(a, "Swift_irr_opt"), "erro Swift (dense mtx col-major)%s \n", P.name.c_str());
+    return 2;
+}
+
+// preprocess: the build's phase times (column sort; blocking = staircase
+// tiles, reside part, categorisation, balance and the kernel schedules)
+// and the product, test/preprocess/main.cu:440-456.
+template <class T>
+static int mode_preprocess(const Args &a, Problem<T> &P) {
+    Operands<T> o;
+    make_operands(P, a.K, o);
+    const float final_ms = swift_time(P, o, SWIFT_ROW_MAJOR, SWIFT_KERNEL_DEFAULT, a.reps,
+                                      "Swift row-major default");
+    const double sort = P.inf.build_ms[0], block = P.inf.build_ms[1] + P.inf.build_ms[2];
+    const double pre = sort + block;
+    printf("Sort %f ms, block %f ms\nPreprocess %f ms, Process %f ms\n", sort, block, pre, final_ms);
+    if (final_ms < 0) {
+        append(out_path<T>(a, "Swift_preprocess"), "erro Swift %s \n", P.name.c_str());
+        return 2;
+    }
+    append(out_path<T>(a, "Swift_preprocess"),
+           "%s m %d n %d nnz %d sort %f ratio %f block %f ratio %f preprocess %f final %f "
+           "overall_time %f \n",
+           P.name.c_str(), P.m, P.n, P.nnz, sort, pre > 0 ? sort / pre : 0.0, block,
+           pre > 0 ? block / pre : 0.0, pre, final_ms, pre + final_ms);
+    return 0;
+}
+
+// mtxinfo: sparsity and the share of empty 32x32 blocks with the product
+// times (test/MTX_info/main.cu:484-493), and the sizes of Swift's regular
+// and irregular parts (test/irr_regular_ratio/main.cu:216).
+template <class T>
+static int mode_mtxinfo(const Args &a, Problem<T> &P) {
+    swift_stats st;
+    if (swift_matrix_stats(P.M, &st, nullptr)) {
+        fprintf(stderr, "swift_matrix_stats: %s\n", swift_last_error());
+        return 1;
+    }
+    printf("sparsity %f empty block ratio %f (%lld of %lld blocks hold entries); regular %d "
+           "(%.4f) irregular %d; tile formats %d/%d/%d; rows <= %d, columns <= %d\n",
+           st.sparsity, st.empty_block_ratio, st.nonempty_blocks, st.blocks, st.regular_nnz,
+           st.regular_ratio, st.reside_nnz, st.tile_format_count[0], st.tile_format_count[1],
+           st.tile_format_count[2], st.max_row_nnz, st.max_col_nnz);
+    Operands<T> o;
+    make_operands(P, a.K, o);
+    const float tv = vendor_time(P, o, a.reps);
+    const float best = min_passing({swift_time(P, o, SWIFT_ROW_MAJOR, SWIFT_KERNEL_DEFAULT, a.reps,
+                                               "Swift row-major default"),
+                                    swift_time(P, o, SWIFT_COL_MAJOR, SWIFT_KERNEL_DEFAULT, a.reps,
+                                               "Swift col-major default")});
+    const std::string out = out_path<T>(a, "Swift_MTX_info");
+    if (best >= 0)
+        append(out, "%s m %d n %d width %d nnz %d rocSPARSE: %f sparsity %f emptyblkratio %f "
+                    "final %f \n",
+               P.name.c_str(), P.m, P.n, a.K, P.nnz, tv, st.sparsity, st.empty_block_ratio, best);
+    else
+        append(out, "erro Swift (dense mtx col-major)%s \n", P.name.c_str());
+    const int sum = st.regular_nnz + st.reside_nnz;
+    std::string ratio = out.substr(0, out.find_last_of('.')) + "_irr_regular_ratio.txt";
+    append(ratio, "%s nnz %d regulatpart %d %f irregularpart %d %f \n", P.name.c_str(), P.nnz,
+           st.regular_nnz, sum ? (double)st.regular_nnz / sum : 0.0, st.reside_nnz,
+           sum ? (double)st.reside_nnz / sum : 0.0);
     return best >= 0 ? 0 : 2;
+}
+
+template <class T>
+static int run(const Args &a) {
+    Problem<T> P;
+    if (int rc = load(a, P)) return rc;
+    if (a.mode == "main") return mode_main(a, P);
+    if (a.mode == "ksweep") return mode_ksweep(a, P);
+    if (a.mode == "cma") return mode_cma(a, P);
+    if (a.mode == "irropt") return mode_irropt(a, P);
+    if (a.mode == "preprocess") return mode_preprocess(a, P);
+    if (a.mode == "mtxinfo") return mode_mtxinfo(a, P);
+    fprintf(stderr, "unknown --mode %s\n", a.mode.c_str());
+    return 2;
 }
 
 int main(int argc, char **argv) {
@@ -302,12 +582,25 @@ int main(int argc, char **argv) {
         else if (s == "--out") a.out = next();
         else if (s == "--gen") a.gen = atoi(next().c_str());
         else if (s == "--scale") a.scale = atof(next().c_str());
-        else pos.push_back(s);
+        else if (s == "--mode") a.mode = next();
+        else if (s == "--ks") {
+            a.ks.clear();
+            std::string v = next();
+            for (size_t p = 0; p < v.size();) {
+                size_t q = v.find(',', p);
+                if (q == std::string::npos) q = v.size();
+                const int k = atoi(v.substr(p, q - p).c_str());
+                if (k <= 0) { fprintf(stderr, "--ks: bad K in %s\n", v.c_str()); return 2; }
+                a.ks.push_back(k);
+                p = q + 1;
+            }
+        } else pos.push_back(s);
     }
     if (!a.gen) {
         if (pos.empty()) {
             fprintf(stderr, "usage: %s -d <device> <A.mtx> [K] [--fp32] [--seed N] [--reps N] "
-                            "[--out FILE] | --gen W [--scale S] [K]\n", argv[0]);
+                            "[--out FILE] [--mode main|ksweep|cma|irropt|preprocess|mtxinfo] "
+                            "[--ks K1,K2,..] | --gen W [--scale S] [K]\n", argv[0]);
             return 2;
         }
         a.path = pos[0];
@@ -320,7 +613,7 @@ int main(int argc, char **argv) {
     hipDeviceProp_t p;
     HIP_OK(hipGetDeviceProperties(&p, a.device));
     printf("---------------------------------------------------------------------------------------------\n");
-    printf("Device [ %i ] %s (%s) @ %4.2f MHz, %s\n", a.device, p.name, p.gcnArchName,
-           p.clockRate * 1e-3f, swift_version());
+    printf("Device [ %i ] %s (%s) @ %4.2f MHz, %s, mode %s\n", a.device, p.name, p.gcnArchName,
+           p.clockRate * 1e-3f, swift_version(), a.mode.c_str());
     return a.fp32 ? run<float>(a) : run<double>(a);
 }

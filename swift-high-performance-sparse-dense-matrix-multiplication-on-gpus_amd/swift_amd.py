@@ -44,7 +44,17 @@ class Info(C.Structure):
                            "sortrowidx", "sortval", "countrow", "segmentoffset",
                            "sortrowindex", "reside_cscptr", "reside_cscrowidx", "reside_val",
                            "colorblk", "blkstart", "blkstop", "blkptr", "val_position",
-                           "val_offset", "rowsorted", "sortsrc", "reside_src")]
+                           "val_offset", "rowsorted", "sortsrc", "reside_src")] + [
+        ("build_ms", C.c_float * 4)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("sparsity", C.c_double), ("empty_block_ratio", C.c_double),
+                ("blocks", C.c_longlong), ("nonempty_blocks", C.c_longlong),
+                ("regular_nnz", C.c_int), ("reside_nnz", C.c_int),
+                ("regular_ratio", C.c_double), ("tilenum", C.c_int),
+                ("tile_format_count", C.c_int * 3), ("max_row_nnz", C.c_int),
+                ("max_col_nnz", C.c_int), ("mean_row_nnz", C.c_double)]
 
 
 # every symbol include/swift_spmm.h declares
@@ -52,7 +62,8 @@ EXPORTS = (
     "swift_last_error", "swift_version", "swift_mmio_allinone", "swift_mmio_allinone_f32",
     "swift_host_free", "swift_generate", "swift_generate_dense", "swift_csr_to_csc",
     "swift_col_sort", "swift_build", "swift_build_host", "swift_destroy",
-    "swift_row_idx_sort", "swift_set_segment_items", "swift_get_info", "swift_permute_dense", "swift_spmm",
+    "swift_row_idx_sort", "swift_set_segment_items", "swift_get_info", "swift_matrix_stats",
+    "swift_permute_dense", "swift_spmm",
     "swift_spmm_timed",
     "swift_spmm_host", "swift_prepare_values", "swift_convert_dense", "swift_spmm_scaled",
     "swift_build_transpose", "swift_sddmm", "swift_backward_fused", "swift_update_values",
@@ -89,6 +100,7 @@ def lib():
         L.swift_destroy.argtypes = [_vp]
         L.swift_destroy.restype = None
         L.swift_get_info.argtypes = [_vp, _P(Info)]
+        L.swift_matrix_stats.argtypes = [_vp, _P(Stats), _vp]
         L.swift_row_idx_sort.argtypes = [_vp, _vp]
         L.swift_set_segment_items.argtypes = [_vp, C.c_int, _vp]
         L.swift_spmm.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, _vp, _vp,
@@ -256,6 +268,12 @@ class SwiftMatrix:
         inf = Info()
         check("swift_get_info", lib().swift_get_info(self.h, C.byref(inf)))
         return inf
+
+    def stats(self):
+        """swift_matrix_stats: sparsity, empty 32x32 block share, part sizes."""
+        st = Stats()
+        check("swift_matrix_stats", lib().swift_matrix_stats(self.h, C.byref(st), _stream()))
+        return st
 
     def arrays(self, names=None):
         """Copy format arrays to host numpy (for bit-exact comparisons)."""

@@ -212,3 +212,28 @@ def test_planted_mistake_is_caught(oracle):
     want = sorted((i, int(g["colidx"][e])) for i in range(int(g["m"]))
                   for e in range(g["rowptr"][i], g["rowptr"][i + 1]))
     assert got != want
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (31, 33), (64, 64), (100, 257), (300, 200), (10, 40)])
+def test_block_stats_brute_force(oracle, shape):
+    """Empty 32x32 blocks (test/MTX_info/main.cu:143-176) against a dense
+    block reduction of the pattern."""
+    m, n = shape
+    mm, nn, rp, ci, _ = matgen.random_csr(m, n, 0.01, 7 + m)
+    blocks, empty, sparsity = oracle.block_stats(m, n, rp, ci)
+    A = matgen.csr_to_dense(m, n, rp, ci, np.ones(len(ci))) != 0
+    rb, cb = -(-m // 32), -(-n // 32)
+    P = np.zeros((rb * 32, cb * 32), bool)
+    P[:m, :n] = A
+    occupied = P.reshape(rb, 32, cb, 32).any(axis=(1, 3))
+    assert blocks == rb * cb
+    assert empty == int((~occupied).sum())
+    assert sparsity == pytest.approx(len(ci) / (m * n))
+
+
+def test_block_stats_known_blocks(oracle):
+    """Entries placed in exactly three blocks of a 64 x 96 matrix (2 x 3 blocks)."""
+    rp = np.cumsum(np.r_[0, [2 if i == 5 else 1 if i == 39 else 0 for i in range(64)]]).astype(np.int32)
+    ci = np.array([0, 95, 40], np.int32)   # row 5: blocks (0,0), (0,2); row 39: block (1,1)
+    blocks, empty, _ = oracle.block_stats(64, 96, rp, ci)
+    assert (blocks, empty) == (6, 3)
