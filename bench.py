@@ -107,6 +107,8 @@ def main():
                     help="also time the batched form over this many generated matrices")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 256 MiB device copy (counter calibration)")
+    ap.add_argument("--kernel", default="DEFAULT",
+                    help="row-major kernel family (DEFAULT, ROWVIEW, WINDOW, FLAT, BALANCED)")
     ap.add_argument("--part", choices=("all", "banded", "zipf"), default="all",
                     help="diagnostics: keep only the banded (even) or the Zipf (odd) rows")
     args = ap.parse_args()
@@ -150,8 +152,9 @@ def main():
     C = torch.empty((m, K), device=dev, dtype=torch.float32)
     stream = torch.cuda.current_stream()
 
+    kern = getattr(S, "KERNEL_" + args.kernel.upper())
     for _ in range(args.warmup):
-        M.spmm(B, out=C)
+        M.spmm(B, out=C, kernel=kern)
     torch.cuda.synchronize()
 
     # whole-job throughput: K steps bracketed by synchronisation
@@ -160,7 +163,7 @@ def main():
     t0 = time.perf_counter()
     e0.record(stream)
     for _ in range(args.steps):
-        M.spmm(B, out=C)
+        M.spmm(B, out=C, kernel=kern)
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -171,7 +174,8 @@ def main():
         (ms_step, wall * 1e3 / args.steps, value))
 
     # per-kernel launch times (HIP events between the launches, same stream)
-    phases = np.array([M.spmm_timed(B, C) for _ in range(max(5, min(args.steps, 20)))])
+    phases = np.array([M.spmm_timed(B, C, kernel=kern)
+                       for _ in range(max(5, min(args.steps, 20)))])
     zero_ms, reg_ms, res_ms = [float(x) for x in np.median(phases, axis=0)]
     log("phases (median ms): zero %.4f regular %.4f reside %.4f" % (zero_ms, reg_ms, res_ms))
     alg = algorithmic_bytes(m, n, nnz, K)
@@ -197,7 +201,7 @@ def main():
                                "8..32 nnz, odd rows Zipf(1.8) <=20k), FP32, K=%d, row-major B "
                                "U[-1,1] in original column order" % (m, n, nnz, K),
                    "rows": m, "cols": n, "nnz": nnz, "K": K, "layout": "row-major",
-                   "rows_kept": args.part,
+                   "rows_kept": args.part, "kernel": args.kernel.upper(),
                    "build_ms": round(build_ms, 1),
                    "phase_ms": {"zero_C": round(zero_ms, 4), "regular": round(reg_ms, 4),
                                 "reside": round(res_ms, 4)}},
@@ -247,6 +251,10 @@ def extras(S, torch, M, d, B, K):
     C = torch.empty((m, K), device="cuda", dtype=torch.float32)
     base = time_call(torch, lambda: M.spmm(B, out=C))
     res["f32_row_ms"] = round(base, 4)
+    for name in ("ROWVIEW", "WINDOW"):
+        k = getattr(S, "KERNEL_" + name)
+        res["f32_row_%s_ms" % name.lower()] = round(
+            time_call(torch, lambda: M.spmm(B, out=C, kernel=k)), 4)
     res["f32_row_tiles_scatter_ms"] = round(time_call(
         torch, lambda: M.spmm(B, out=C, kernel=S.KERNEL_BALANCED), reps=3, warm=1), 4)
     perm = torch.tensor(M.arrays(["perm"])["perm"].astype(np.int64), device="cuda")
@@ -274,6 +282,10 @@ def extras(S, torch, M, d, B, K):
     bf = time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16))
     res["bf16_row_ms"] = round(bf, 4)
     res["bf16_speedup_vs_f32"] = round(base / bf, 3)
+    for name in ("ROWVIEW", "WINDOW"):
+        k = getattr(S, "KERNEL_" + name)
+        res["bf16_row_%s_ms" % name.lower()] = round(
+            time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16, kernel=k)), 4)
     dC = torch.randn((m, K), device="cuda")
     bw = time_call(torch, lambda: M.backward_fused(dC, B), reps=5)
     res["fwd_bwd_over_fwd"] = round((base + bw) / base, 3)

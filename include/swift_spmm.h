@@ -55,9 +55,12 @@ typedef enum {
                                     through its row view (merge-path balanced), one
                                     row at a time per wave; reside part in balance
                                     blocks                                           */
-    SWIFT_KERNEL_FLAT = 6        /* row-major: row view walked by independent lane
+    SWIFT_KERNEL_FLAT = 6,       /* row-major: row view walked by independent lane
                                     groups, several entries in flight whatever the
                                     row lengths; reside part in balance blocks       */
+    SWIFT_KERNEL_WINDOW = 7      /* row-major: ROWVIEW walk by workgroups of 16
+                                    segments that first stage the B rows their
+                                    entries reuse most (a column window) in LDS    */
 } swift_kernel;
 
 /* swift_spmm flags */

@@ -140,6 +140,30 @@ __device__ __forceinline__ void load_b_buf(__amdgpu_buffer_rsrc_t r, unsigned of
     }
 }
 
+// B rows staged in LDS by a workgroup (window walk): rows [w0, w0 + wlen) of
+// the launch's K slice, rb bytes apart from lds on.  wlen == 0: no window.
+struct Window {
+    unsigned char *lds;
+    int w0, wlen;
+    unsigned rb;
+};
+
+// VEC elements of a staged B row at LDS byte address p, widened.
+template <class TB, int VEC, class TC>
+__device__ __forceinline__ void load_b_lds(const unsigned char *p, TC (&o)[VEC]) {
+    constexpr int BYTES = (int)sizeof(TB) * VEC;
+    static_assert(BYTES == 16 || BYTES == 8, "window walk needs 8- or 16-byte lane loads");
+    if constexpr (BYTES == 16) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(p);
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+        widen_words<TB, TC>(w, 4, o);
+    } else {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p);
+        const unsigned w[2] = {v.x, v.y};
+        widen_words<TB, TC>(w, 2, o);
+    }
+}
+
 // Streaming accesses.  The row view's (column, value, row end) arrays and the
 // finished output rows are touched once per launch; loading and storing them
 // with the non-temporal bit (nt) puts their lines first in line for eviction,
@@ -328,6 +352,45 @@ __device__ __forceinline__ double bperm_val(int addr, double x) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// One step of the buffer-load walk: UNROLL slots of EPS entries each; slot u
+// of lane group h takes the staged (column, value) of lane base/4 + u*EPS.
+// A slot wholly past the row's end issues nothing (a wave-uniform branch); a
+// partly used one reads zeros.
+template <class TB, class TC, int VEC, int Q, int LPE, bool MASK, int UNROLL>
+__device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rsrc_t rsrc,
+                                            int base, int j, int nb, int h, int sub, int mc,
+                                            TC mv, TC (&acc)[Q][VEC]) {
+    constexpr int EPS = 64 / LPE;
+    TC b[UNROLL][Q][VEC];
+    TC vv[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+        if (u > 0 && j + u * EPS >= nb) break;
+        const int idx = j + u * EPS + h;
+        // one base address per step, the slot in the instruction offset
+        const int c = __builtin_amdgcn_ds_bpermute(base + ((u * EPS) << 2), mc);
+        const TC vs = bperm_val(base + ((u * EPS) << 2), mv);
+        const bool ok = idx < nb;
+        vv[u] = ok ? vs : (TC)0;
+        const unsigned roff = __umul24((unsigned)c, a.bstride);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int k0 = q * LPE * VEC + sub * VEC;
+            const bool in = ok && (!MASK || k0 < a.kcount);
+            load_b_buf<TB, VEC, TC>(
+                rsrc, in ? roff + (unsigned)(k0 * (int)sizeof(TB)) : OOB_OFFSET, b[u][q]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+        if (u > 0 && j + u * EPS >= nb) break;
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+            for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * b[u][q][v];
+    }
+}
+
 // Merge-path walk of the row view: each wave owns one segment of W items
 // (rows + entries), so every wave does the same amount of work whatever the
 // row lengths.  The segment's entries are contiguous in the row view, so they
@@ -345,8 +408,14 @@ __device__ __forceinline__ double bperm_val(int addr, double x) {
 #define SWIFT_UNROLL_WIDE 4
 #endif
 
-template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF>
-__device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
+//
+// WIN: the workgroup has staged a window of B rows in LDS (win).  A step
+// whose used slots all fall inside the window reads them from LDS; any other
+// step reads B from memory as without a window.
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF,
+          bool WIN = false>
+__device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window win = Window{}) {
+    static_assert(!WIN || (BUF && !MASK), "the window walk is a buffer-load, unmasked walk");
     constexpr int EPS = 64 / LPE;
     // B-row loads in flight per lane group and step
     constexpr int UNROLL = sizeof(TB) == 2 ? SWIFT_UNROLL_NARROW : SWIFT_UNROLL_WIDE;
@@ -401,28 +470,57 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
             for (int j = 0; j < nb; j += STEP) {
                 TC b[UNROLL][Q][VEC];
                 TC vv[UNROLL];
-                if constexpr (BUF) {
-                    // slot u reads staged lane off + j + u*EPS + h: one base
-                    // address per step, the slot in the instruction offset
+                if constexpr (WIN) {
                     const int base = (off + j + h) << 2;
+                    int cu[UNROLL];
+                    bool oku[UNROLL];
+                    bool far = false;
 #pragma unroll
                     for (int u = 0; u < UNROLL; u++) {
-                        // a slot wholly past the row's end issues nothing (a
-                        // wave-uniform branch); a partly used one reads zeros
+                        cu[u] = 0;
+                        oku[u] = false;
+                        vv[u] = (TC)0;
                         if (u > 0 && j + u * EPS >= nb) break;
-                        const int idx = j + u * EPS + h;
                         const int c = __builtin_amdgcn_ds_bpermute(base + ((u * EPS) << 2), mc);
                         const TC vs = bperm_val(base + ((u * EPS) << 2), mv);
-                        const bool ok = idx < nb;
+                        const bool ok = j + u * EPS + h < nb;
+                        cu[u] = c;
+                        oku[u] = ok;
                         vv[u] = ok ? vs : (TC)0;
-                        const unsigned roff = __umul24((unsigned)c, a.bstride);
+                        far = far || (ok && (unsigned)(c - win.w0) >= (unsigned)win.wlen);
+                    }
+                    // wave-uniform: every used slot of the step is staged
+                    const bool staged = win.wlen > 0 && __ballot(far) == 0;
+                    if (staged) {
 #pragma unroll
-                        for (int q = 0; q < Q; q++) {
-                            const int k0 = q * LPE * VEC + sub * VEC;
-                            const bool in = ok && (!MASK || k0 < a.kcount);
-                            load_b_buf<TB, VEC, TC>(
-                                rsrc, in ? roff + (unsigned)(k0 * (int)sizeof(TB)) : OOB_OFFSET,
-                                b[u][q]);
+                        for (int u = 0; u < UNROLL; u++) {
+                            if (u > 0 && j + u * EPS >= nb) break;
+                            const unsigned char *p =
+                                win.lds + (unsigned)(cu[u] - win.w0) * win.rb;
+#pragma unroll
+                            for (int q = 0; q < Q; q++) {
+                                const int k0 = q * LPE * VEC + sub * VEC;
+                                if (oku[u]) {
+                                    load_b_lds<TB, VEC, TC>(p + k0 * (int)sizeof(TB), b[u][q]);
+                                } else {
+#pragma unroll
+                                    for (int v = 0; v < VEC; v++) b[u][q][v] = (TC)0;
+                                }
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < UNROLL; u++) {
+                            if (u > 0 && j + u * EPS >= nb) break;
+                            const unsigned roff = __umul24((unsigned)cu[u], a.bstride);
+#pragma unroll
+                            for (int q = 0; q < Q; q++) {
+                                const int k0 = q * LPE * VEC + sub * VEC;
+                                load_b_buf<TB, VEC, TC>(
+                                    rsrc, oku[u] ? roff + (unsigned)(k0 * (int)sizeof(TB))
+                                                 : OOB_OFFSET,
+                                    b[u][q]);
+                            }
                         }
                     }
 #pragma unroll
@@ -433,6 +531,11 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid) {
 #pragma unroll
                             for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * b[u][q][v];
                     }
+                    continue;
+                }
+                if constexpr (BUF) {
+                    gather_step<TB, TC, VEC, Q, LPE, MASK, UNROLL>(
+                        a, rsrc, (off + j + h) << 2, j, nb, h, sub, mc, mv, acc);
                     continue;
                 }
 #pragma unroll

@@ -85,6 +85,143 @@ __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
     rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF>(a, sid);
 }
 
+// Window walk.  One workgroup of WIN_WAVES waves takes WIN_WAVES consecutive
+// segments (one per wave) and first picks the B rows its entries reuse most:
+// WIN_CAND candidate windows of wcap rows (LDS bytes / B row bytes), centred
+// on the columns of evenly spaced entries, are scored by how many of the
+// group's entries they hold; the best is shrunk to the span of the columns it
+// holds and, if every staged row is used WIN_REUSE times on average, copied
+// into LDS once.  The waves then walk their segments as k_rowview_row does,
+// reading a step's B rows from LDS when all of them are staged.  Rows of
+// banded or locally clustered matrices thus cost one L2 read per workgroup
+// instead of one per entry; scattered rows keep the plain walk's loads.
+// Two workgroups (32 waves, the plain walk's occupancy) share a CU's 160 KB.
+#ifndef SWIFT_WIN_WAVES
+#define SWIFT_WIN_WAVES 16
+#endif
+#ifndef SWIFT_WIN_LDS_KB
+#define SWIFT_WIN_LDS_KB 76
+#endif
+#ifndef SWIFT_WIN_MIN_ENTRIES
+#define SWIFT_WIN_MIN_ENTRIES 256
+#endif
+constexpr int WIN_WAVES = SWIFT_WIN_WAVES;
+constexpr int WIN_THREADS = WIN_WAVES * 64;
+constexpr unsigned WIN_LDS_BYTES = SWIFT_WIN_LDS_KB * 1024u;
+constexpr int WIN_CAND = 32;
+constexpr int WIN_MIN_ENTRIES = SWIFT_WIN_MIN_ENTRIES;
+constexpr int WIN_REUSE = 2;
+constexpr int WIN_MIN_ROWS = 16;
+
+template <class TV, class TB, class TC, int VEC, int Q, int LPE>
+__global__ __launch_bounds__(WIN_THREADS, 8) void k_rowview_win(RowArgs a) {
+    __shared__ __align__(16) unsigned char wbuf[WIN_LDS_BYTES];
+    __shared__ int cand[WIN_CAND], cnt[WIN_CAND], sel[4];
+    const unsigned g = xcd_block(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int s0 = (int)g * WIN_WAVES;
+    const int s1 = min(s0 + WIN_WAVES, a.nseg);
+    const int E0 = a.seg[s0].y, E1 = a.seg[s1].y;
+    const int n = E1 - E0;
+    const unsigned rb = (unsigned)(a.kcount * (int)sizeof(TB));
+    const int wcap = (int)(WIN_LDS_BYTES / rb);
+    Window win{wbuf, 0, 0, rb};
+    // every branch below that holds a barrier is uniform over the workgroup
+    if (n >= WIN_MIN_ENTRIES) {
+        auto col = [&](int e) {
+            const int c = a.rv_col[e];
+            return a.inv_perm ? a.inv_perm[c] : c;
+        };
+        if (tid < WIN_CAND) {
+            const int c = col(E0 + (int)(((long long)tid * n) / WIN_CAND));
+            cand[tid] = max(0, c - wcap / 2);
+            cnt[tid] = 0;
+        }
+        if (tid == 0) {
+            sel[2] = 0x7FFFFFFF;
+            sel[3] = -1;
+        }
+        __syncthreads();
+        const int iters = (n + WIN_THREADS - 1) / WIN_THREADS;
+        int mine = 0;  // lane i: entries of this wave inside candidate i
+        for (int it = 0; it < iters; it++) {
+            const int e = E0 + it * WIN_THREADS + tid;
+            const bool live = e < E1;
+            const int c = live ? col(e) : 0;
+#pragma unroll 8
+            for (int i = 0; i < WIN_CAND; i++) {
+                const bool in = live && (unsigned)(c - cand[i]) < (unsigned)wcap;
+                const int pop = __popcll(__ballot(in));
+                mine += lane == i ? pop : 0;
+            }
+        }
+        if (lane < WIN_CAND && mine) atomicAdd(&cnt[lane], mine);
+        __syncthreads();
+        if (wave == 0) {
+            int bc = lane < WIN_CAND ? cnt[lane] : -1, bi = lane;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const int oc = __shfl_xor(bc, o, 64), oi = __shfl_xor(bi, o, 64);
+                if (oc > bc || (oc == bc && oi < bi)) {
+                    bc = oc;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                sel[0] = cand[bi];
+                sel[1] = bc;
+            }
+        }
+        __syncthreads();
+        const int ws = __builtin_amdgcn_readfirstlane(sel[0]);
+        int lo = 0x7FFFFFFF, hi = -1;
+        for (int it = 0; it < iters; it++) {
+            const int e = E0 + it * WIN_THREADS + tid;
+            if (e < E1) {
+                const int c = col(e);
+                if ((unsigned)(c - ws) < (unsigned)wcap) {
+                    lo = min(lo, c);
+                    hi = max(hi, c);
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o, 64));
+            hi = max(hi, __shfl_xor(hi, o, 64));
+        }
+        if (lane == 0 && hi >= 0) {
+            atomicMin(&sel[2], lo);
+            atomicMax(&sel[3], hi);
+        }
+        __syncthreads();
+        const int w0 = __builtin_amdgcn_readfirstlane(sel[2]);
+        const int wh = __builtin_amdgcn_readfirstlane(sel[3]);
+        const int wn = __builtin_amdgcn_readfirstlane(sel[1]);
+        const int wl = wh - w0 + 1;
+        if (wh >= 0 && wn >= WIN_REUSE * wl) {
+            // B rows [w0, w0 + wl) (wl <= wcap, so wl * rb <= WIN_LDS_BYTES)
+            // in 16-byte pieces through the walk's buffer resource
+            const __amdgpu_buffer_rsrc_t rsrc =
+                make_rsrc(static_cast<const TB *>(a.B) + a.kbase, a.bbytes);
+            const int cpr = (int)(rb >> 4);
+            const int chunks = wl * cpr;
+            for (int ch = tid; ch < chunks; ch += WIN_THREADS) {
+                const int r = ch / cpr, p = ch - r * cpr;
+                const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(
+                    rsrc, (unsigned)(w0 + r) * a.bstride + (unsigned)(p << 4), 0, 0);
+                *reinterpret_cast<u32x4 *>(wbuf + ((unsigned)ch << 4)) = v;
+            }
+            win.w0 = w0;
+            win.wlen = wl;
+            __syncthreads();
+        }
+    }
+    // wave-uniform segment id, kept in a scalar register
+    const int sid = __builtin_amdgcn_readfirstlane(s0 + wave);
+    if (sid < a.nseg) rowview_walk<TV, TB, TC, VEC, Q, LPE, false, true, true>(a, sid, win);
+}
+
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
 __global__ __launch_bounds__(256) void k_reside_row(RowArgs a) {
     const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
@@ -318,10 +455,26 @@ static std::vector<std::pair<int, RowTarget>> plan_row_slices(int K, int vec, bo
     return out;
 }
 
-enum RowKind { ROW_TILES = 0, ROW_RESIDE = 1, ROW_VIEW = 2, ROW_FLAT = 3 };
+enum RowKind { ROW_TILES = 0, ROW_RESIDE = 1, ROW_VIEW = 2, ROW_FLAT = 3, ROW_WIN = 4 };
 
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
 static void launch_row_shape(const RowArgs &a, RowKind kind, hipStream_t s) {
+    if (kind == ROW_WIN) {
+        // the window walk needs buffer loads, whole 8/16-byte lane loads, one
+        // chunk per lane (K slices up to 64 lanes x one load: wider rows
+        // leave too few rows in the window) and room for WIN_MIN_ROWS B rows;
+        // otherwise the plain walk runs
+        if constexpr (!MASK && Q == 1 && (sizeof(TB) * VEC == 16 || sizeof(TB) * VEC == 8)) {
+            const unsigned rb = (unsigned)(a.kcount * (int)sizeof(TB));
+            if (a.nseg && a.bstride && WIN_LDS_BYTES / rb >= (unsigned)WIN_MIN_ROWS) {
+                hipLaunchKernelGGL((k_rowview_win<TV, TB, TC, VEC, Q, LPE>),
+                                   dim3(ceil_div((size_t)a.nseg, WIN_WAVES)), dim3(WIN_THREADS),
+                                   0, s, a);
+                return;
+            }
+        }
+        kind = ROW_VIEW;
+    }
     if (kind == ROW_FLAT) {
         constexpr int U = SWIFT_FLAT_U < LPE ? SWIFT_FLAT_U : LPE;
         if (a.nseg)
@@ -423,7 +576,9 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     const bool aligned = (K % V) == 0 && ((uintptr_t)B % 16) == 0 && ((uintptr_t)C % 16) == 0;
     const auto slices = plan_row_slices(K, V, aligned);
     const bool flat = kernel == SWIFT_KERNEL_FLAT;
-    const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW || flat;
+    const bool winw = kernel == SWIFT_KERNEL_WINDOW;
+    const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW || flat ||
+                      winw;
     if (view) {
         if (!a.accumulate && nsplit) {
             hipLaunchKernelGGL(k_zero_rows<TC>, dim3(ceil_div((size_t)nsplit * 64, 256)),
@@ -446,7 +601,8 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
         a.kcount = t.mask ? std::min(64, K - sl.first) : t.vec * t.lpe * t.q;
         a.bstride = buf ? (unsigned)(K * sizeof(TB)) : 0u;
         a.bbytes = buf ? (unsigned)(((size_t)M->n * K - sl.first) * sizeof(TB)) : 0u;
-        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, flat ? ROW_FLAT : view ? ROW_VIEW : ROW_TILES, s)));
+        const RowKind kind = flat ? ROW_FLAT : winw ? ROW_WIN : view ? ROW_VIEW : ROW_TILES;
+        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, kind, s)));
     }
     mark(2, s);
     for (auto &sl : slices) {
@@ -623,12 +779,13 @@ int spmm_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dty
         set_error("SWIFT_KERNEL_ROWSORT is a column-major kernel");
         return SWIFT_ERR_UNSUPPORTED;
     }
-    if ((kernel == SWIFT_KERNEL_ROWVIEW || kernel == SWIFT_KERNEL_FLAT) &&
+    if ((kernel == SWIFT_KERNEL_ROWVIEW || kernel == SWIFT_KERNEL_FLAT ||
+         kernel == SWIFT_KERNEL_WINDOW) &&
         layout == SWIFT_COL_MAJOR) {
-        set_error("SWIFT_KERNEL_ROWVIEW / SWIFT_KERNEL_FLAT are row-major kernels");
+        set_error("SWIFT_KERNEL_ROWVIEW / FLAT / WINDOW are row-major kernels");
         return SWIFT_ERR_UNSUPPORTED;
     }
-    if ((int)kernel < 0 || (int)kernel > (int)SWIFT_KERNEL_FLAT) {
+    if ((int)kernel < 0 || (int)kernel > (int)SWIFT_KERNEL_WINDOW) {
         set_error("swift_spmm: unknown kernel %d", (int)kernel);
         return SWIFT_ERR_ARG;
     }

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libswift_amd.so")
 F32, F64, BF16, F16 = 0, 1, 2, 3
 ROW_MAJOR, COL_MAJOR = 0, 1
 KERNEL_DEFAULT, KERNEL_BALANCED, KERNEL_UNBALANCED, KERNEL_KCHUNK, KERNEL_ROWSORT, KERNEL_ROWVIEW, \
-    KERNEL_FLAT = range(7)
+    KERNEL_FLAT, KERNEL_WINDOW = range(8)
 B_SORTED, ACCUMULATE = 0x1, 0x2
 
 STATUS = {0: "OK", -1: "ERR_OPEN", -2: "ERR_BANNER", -3: "ERR_DATA", -4: "ERR_SIZE",

@@ -19,6 +19,9 @@ def cases():
         # the benchmark matrix's structure at 1/250 scale: even rows banded,
         # odd rows Zipf lengths with uniform columns
         "banded_mixed_4000": matgen.banded_mixed_csr(4000, 4000, 21),
+        # every row banded (bandwidth 24): workgroups of the window walk stage
+        # their B rows in LDS
+        "banded_3000": matgen.banded_mixed_csr(3000, 3000, 22, bw=24, mixed=False),
     }
     # uniform columns, N % 32 == 0: no irregular part at all
     m = n = 64

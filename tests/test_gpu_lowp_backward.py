@@ -27,15 +27,18 @@ def setup(swift, oracle, name, K, seed=0):
     return M, B, C64, absC
 
 
+ROW_KERNELS = {"ROW_FLAT": "KERNEL_FLAT", "ROW_WIN": "KERNEL_WINDOW"}
+
+
 @pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "equalrows_100x70",
-                                  "narrow_50x20"])
+                                  "narrow_50x20", "banded_3000"])
 @pytest.mark.parametrize("K", [32, 128, 256])
-@pytest.mark.parametrize("layout", ["ROW", "COL", "ROW_FLAT"])
+@pytest.mark.parametrize("layout", ["ROW", "COL", "ROW_FLAT", "ROW_WIN"])
 def test_bf16_storage(swift, oracle, cuda, name, K, layout):
     import torch
     M, B, C64, absC = setup(swift, oracle, name, K)
     lay = swift.COL_MAJOR if layout == "COL" else swift.ROW_MAJOR
-    kernel = swift.KERNEL_FLAT if layout == "ROW_FLAT" else swift.KERNEL_DEFAULT
+    kernel = getattr(swift, ROW_KERNELS.get(layout, "KERNEL_DEFAULT"))
     M.prepare_values(swift.BF16)
     Bt = torch.tensor(B if lay == swift.ROW_MAJOR else B.T.copy(), device="cuda")
     Bh = swift.convert_dense(Bt, swift.BF16, layout=lay)
@@ -49,14 +52,14 @@ def test_bf16_storage(swift, oracle, cuda, name, K, layout):
     M.close()
 
 
-@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100"])
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "banded_3000"])
 @pytest.mark.parametrize("K", [32, 128])
-@pytest.mark.parametrize("layout", ["ROW", "COL", "ROW_FLAT"])
+@pytest.mark.parametrize("layout", ["ROW", "COL", "ROW_FLAT", "ROW_WIN"])
 def test_f16_scaled_storage(swift, oracle, cuda, name, K, layout):
     import torch
     M, B, C64, absC = setup(swift, oracle, name, K)
     lay = swift.COL_MAJOR if layout == "COL" else swift.ROW_MAJOR
-    kernel = swift.KERNEL_FLAT if layout == "ROW_FLAT" else swift.KERNEL_DEFAULT
+    kernel = getattr(swift, ROW_KERNELS.get(layout, "KERNEL_DEFAULT"))
     # widen the dynamic range so that the per-column scales matter
     B = B * np.logspace(-3, 3, K, dtype=np.float32)[None, :]
     C64, absC = oracle.spmm_f64(*CASES[name][:4], np.asarray(CASES[name][4], np.float32)

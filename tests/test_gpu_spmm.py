@@ -55,7 +55,8 @@ def test_col_major(swift, oracle, cuda, name, K, dt):
     assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
 
 
-@pytest.mark.parametrize("kernel", ["BALANCED", "UNBALANCED", "KCHUNK", "ROWVIEW", "FLAT"])
+@pytest.mark.parametrize("kernel", ["BALANCED", "UNBALANCED", "KCHUNK", "ROWVIEW", "FLAT",
+                                    "WINDOW"])
 @pytest.mark.parametrize("layout", ["ROW", "COL"])
 @pytest.mark.parametrize("name", ["densecol_400x100", "equalrows_100x70", "powerlaw_3000x2500",
                                   "empty_10x40", "uniform_64x64"])
@@ -63,7 +64,7 @@ def test_kernel_variants(swift, oracle, cuda, kernel, layout, name):
     print("case", name, kernel, layout, flush=True)
     if layout == "ROW" and kernel == "KCHUNK":
         pytest.skip("K chunks are a column-major grid dimension")
-    if layout == "COL" and kernel in ("ROWVIEW", "FLAT"):
+    if layout == "COL" and kernel in ("ROWVIEW", "FLAT", "WINDOW"):
         pytest.skip("the row-view walks are row-major kernels")
     k = getattr(swift, "KERNEL_" + kernel)
     lay = swift.ROW_MAJOR if layout == "ROW" else swift.COL_MAJOR
@@ -112,12 +113,12 @@ def test_b_sorted_flag(swift, oracle, cuda, layout, dt):
     assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
 
 
-@pytest.mark.parametrize("kernel", ["DEFAULT", "BALANCED", "ROWVIEW", "FLAT"])
+@pytest.mark.parametrize("kernel", ["DEFAULT", "BALANCED", "ROWVIEW", "FLAT", "WINDOW"])
 @pytest.mark.parametrize("K", [32, 100])
 @pytest.mark.parametrize("layout", ["ROW", "COL"])
 def test_accumulate_flag(swift, oracle, cuda, kernel, K, layout):
     import torch
-    if layout == "COL" and kernel in ("ROWVIEW", "FLAT"):
+    if layout == "COL" and kernel in ("ROWVIEW", "FLAT", "WINDOW"):
         pytest.skip("the row-view walks are row-major kernels")
     m, n, rp, ci, v = CASES["powerlaw_3000x2500"]
     vv = np.asarray(v, np.float32)
@@ -186,7 +187,7 @@ def test_rowview_col_major_rejected(swift, cuda):
     import torch
     m, n, rp, ci, v = CASES["random_300x200"]
     M = swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float32)
-    for k in (swift.KERNEL_ROWVIEW, swift.KERNEL_FLAT):
+    for k in (swift.KERNEL_ROWVIEW, swift.KERNEL_FLAT, swift.KERNEL_WINDOW):
         with pytest.raises(swift.SwiftError):
             M.spmm(torch.zeros((32, n), device="cuda"), layout=swift.COL_MAJOR, kernel=k)
     with pytest.raises(swift.SwiftError):
@@ -214,6 +215,52 @@ def test_flat_walk(swift, oracle, cuda, name, K, W):
     C = out.cpu().numpy()
     assert not np.isnan(C).any()
     assert gpu_common.rel_err(C, C64, absC) <= 1e-5
+    M.close()
+
+
+@pytest.mark.parametrize("W", [24, 96, 512])
+@pytest.mark.parametrize("K", [32, 100, 128, 256])
+@pytest.mark.parametrize("name", list(CASES))
+def test_window_walk(swift, oracle, cuda, name, K, W):
+    """Workgroups of 16 segments of W items that stage their most reused B
+    rows in LDS (K = 100 runs three K slices, two of them windowed, at
+    nonzero K offsets); C starts as NaN so a row left unwritten shows up."""
+    import torch
+    print("case", name, K, W, flush=True)
+    m, n, rp, ci, v = CASES[name]
+    vv = np.asarray(v, np.float32)
+    B = np.random.default_rng(K + W).uniform(-1, 1, (n, K)).astype(np.float32)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=torch.float32)
+    M.set_segment_items(W)
+    out = torch.full((m, K), float("nan"), device="cuda")
+    M.spmm(torch.tensor(B, device="cuda"), out=out, kernel=swift.KERNEL_WINDOW)
+    C = out.cpu().numpy()
+    assert not np.isnan(C).any()
+    assert gpu_common.rel_err(C, C64, absC) <= 1e-5
+    M.close()
+
+
+@pytest.mark.parametrize("name", ["banded_3000", "banded_mixed_4000", "powerlaw_3000x2500"])
+@pytest.mark.parametrize("K", [16, 64, 128])
+def test_window_walk_f64_sorted(swift, oracle, cuda, name, K):
+    """FP64 window walk, B in original and in sorted column order (the
+    window is then chosen in sorted order)."""
+    import torch
+    print("case", name, K, flush=True)
+    m, n, rp, ci, v = CASES[name]
+    vv = np.asarray(v, np.float64)
+    B = np.random.default_rng(K).uniform(-1, 1, (n, K))
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv, B)
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=torch.float64)
+    perm = torch.tensor(M.arrays(["perm"])["perm"].astype(np.int64), device="cuda")
+    Bt = torch.tensor(B, device="cuda")
+    out = torch.full((m, K), float("nan"), device="cuda", dtype=torch.float64)
+    M.spmm(Bt, out=out, kernel=swift.KERNEL_WINDOW)
+    assert gpu_common.rel_err(out.cpu().numpy(), C64, absC) <= 1e-12
+    out.fill_(float("nan"))
+    M.spmm(Bt[perm].contiguous(), out=out, kernel=swift.KERNEL_WINDOW, flags=swift.B_SORTED)
+    assert gpu_common.rel_err(out.cpu().numpy(), C64, absC) <= 1e-12
     M.close()
 
 
