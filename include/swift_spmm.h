@@ -197,7 +197,10 @@ int swift_get_info(const swift_matrix *mat, swift_info *info);
  * (test/MTX_info/main.cu:143-176, :484), the sizes of Swift's regular (tile)
  * and irregular (reside) parts (test/irr_regular_ratio/main.cu:196-218, which
  * counts tiles of its own variant of the tiling; here nonzeros of the two
- * parts), the tile formats of block_catgory and the row/column extremes. */
+ * parts), the tile formats of block_catgory and the row/column extremes.
+ * The block density figures (dense_blocks, dense_block_nnz, max_block_nnz)
+ * have no counterpart in the reference: they say how much of A would fit
+ * dense matrix-core tiles at least half full. */
 typedef struct {
     double sparsity;            /* nnz / (m * n)                                */
     double empty_block_ratio;   /* 32x32 blocks without an entry / all blocks   */
@@ -208,6 +211,9 @@ typedef struct {
     int tile_format_count[3];   /* tiles of format 0, 1, 2                      */
     int max_row_nnz, max_col_nnz;
     double mean_row_nnz;
+    long long dense_blocks;     /* 32x32 blocks with >= 512 entries (half full)  */
+    long long dense_block_nnz;  /* entries inside those blocks                   */
+    int max_block_nnz;          /* entries of the fullest 32x32 block            */
 } swift_stats;
 int swift_matrix_stats(const swift_matrix *mat, swift_stats *stats, swift_stream_t stream);
 

@@ -251,6 +251,15 @@ def block_stats(m, n, rowptr, colidx):
     return blocks.value, empty.value, (nnz / (m * n) if m and n else 0.0)
 
 
+def block_density(m, n, rowptr, colidx, thr=512):
+    """(32x32 blocks with >= thr entries, their entries, largest block count)."""
+    d, dz, mx = C.c_longlong(), C.c_longlong(), C.c_int()
+    lib().oracle_block_density(m, n, _ip(np.ascontiguousarray(rowptr, np.int32)),
+                               _ip(np.ascontiguousarray(colidx, np.int32)), thr,
+                               C.byref(d), C.byref(dz), C.byref(mx))
+    return d.value, dz.value, mx.value
+
+
 # ------------------------------------------------------------------ _ref --
 
 class _RefView(C.Structure):

@@ -604,3 +604,31 @@ void oracle_block_stats(int m, int n, const int *rowptr, const int *colidx,
     *blocks = total;
     *empty = e;
 }
+
+/* Entry counts of the 32x32 blocks as above: blocks holding at least thr
+ * entries, the entries they hold, and the largest count (the density figures
+ * behind the choice of dense matrix-core tiles; no counterpart in the
+ * reference). */
+void oracle_block_density(int m, int n, const int *rowptr, const int *colidx, int thr,
+                          long long *dense, long long *dense_nnz, int *max_count) {
+    const long long rows = m % 32 == 0 ? m / 32 : m / 32 + 1;
+    const long long cols = n % 32 == 0 ? n / 32 : n / 32 + 1;
+    const long long total = rows * cols;
+    int *count = (int *)calloc(total > 0 ? (size_t)total : 1, sizeof(int));
+    for (int i = 0; i < m; i++)
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++)
+            count[(long long)(i / 32) * cols + colidx[j] / 32]++;
+    long long d = 0, dz = 0;
+    int mx = 0;
+    for (long long b = 0; b < total; b++) {
+        if (count[b] > mx) mx = count[b];
+        if (count[b] >= thr) {
+            d++;
+            dz += count[b];
+        }
+    }
+    free(count);
+    *dense = d;
+    *dense_nnz = dz;
+    *max_count = mx;
+}

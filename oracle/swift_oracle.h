@@ -116,6 +116,8 @@ void oracle_sddmm_csr_f64(int m, int n, const int *rowptr, const int *colidx,
  * *blocks = ceil(m/32)*ceil(n/32), *empty = blocks holding no entry. */
 void oracle_block_stats(int m, int n, const int *rowptr, const int *colidx,
                         long long *blocks, long long *empty);
+void oracle_block_density(int m, int n, const int *rowptr, const int *colidx, int thr,
+                          long long *dense, long long *dense_nnz, int *max_count);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,55 @@ __global__ __launch_bounds__(256) void k_nonempty_blocks(int m, int nbc,
     if (threadIdx.x == 0 && t) atomicAdd(count, t);
 }
 
+// Entry count of every 32x32 block of a band of 32 rows, SCNT column blocks
+// per pass in LDS counters: blocks with at least DENSE_BLOCK entries (half
+// full), the entries they hold, and the largest count.  These are the density
+// figures that decide whether dense matrix-core tiles could pay.
+constexpr int SCNT = 8192;         // counters per pass, 32 KiB
+constexpr int DENSE_BLOCK = 512;   // entries of a half-full 32x32 block
+
+__global__ __launch_bounds__(256) void k_block_density(int m, int nbc,
+                                                       const int *__restrict__ rowptr,
+                                                       const int *__restrict__ colidx,
+                                                       unsigned long long *__restrict__ out,
+                                                       int *__restrict__ maxcnt) {
+    __shared__ unsigned cnt[SCNT];
+    const int r0 = blockIdx.x * SBLK, r1 = min(m, r0 + SBLK);
+    const int e0 = rowptr[r0], e1 = rowptr[r1];
+    if (e0 == e1) return;  // uniform over the workgroup: an empty band
+    unsigned long long nd = 0, nz = 0;
+    int mx = 0;
+    for (int base = 0; base < nbc; base += SCNT) {
+        const int words = min(SCNT, nbc - base);
+        for (int w = threadIdx.x; w < words; w += blockDim.x) cnt[w] = 0u;
+        __syncthreads();
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+            const int cb = colidx[e] / SBLK - base;
+            if (cb >= 0 && cb < words) atomicAdd(&cnt[cb], 1u);
+        }
+        __syncthreads();
+        for (int w = threadIdx.x; w < words; w += blockDim.x) {
+            const int c = (int)cnt[w];
+            mx = max(mx, c);
+            if (c >= DENSE_BLOCK) {
+                nd++;
+                nz += (unsigned long long)c;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0 && mx) atomicMax(maxcnt, mx);
+    const unsigned long long tnd = block_sum_u64(nd);
+    __syncthreads();  // block_sum_u64's partials are reused by the next sum
+    const unsigned long long tnz = block_sum_u64(nz);
+    if (threadIdx.x == 0 && tnd) {
+        atomicAdd(out, tnd);
+        atomicAdd(out + 1, tnz);
+    }
+}
+
 // Longest row (entries per row from the kept CSR row pointer).
 __global__ void k_max_row(int m, const int *__restrict__ rowptr, int *__restrict__ out) {
     int best = 0;
@@ -95,11 +144,16 @@ int matrix_stats_impl(const swift_matrix *M, swift_stats *o, hipStream_t s) {
     Buffer buf;
     SWIFT_TRY(buf.alloc(64));
     SWIFT_TRY(fill_bytes_zero(buf.ptr, 64, s));
+    // [0] non-empty blocks, [1] dense blocks, [2] their entries (u64); then
+    // ints: [0] max row, [1..3] tile formats, [4] max block count
     unsigned long long *cnt = buf.as<unsigned long long>();
-    int *ints = reinterpret_cast<int *>(cnt + 1);  // [0] max row, [1..3] tile formats
+    int *ints = reinterpret_cast<int *>(cnt + 3);
     if (nbr && nbc && M->nnz) {
         hipLaunchKernelGGL(k_nonempty_blocks, dim3((unsigned)nbr), dim3(256), 0, s, M->m,
                            (int)nbc, M->csr_rowptr, M->csr_colidx, cnt);
+        SWIFT_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_block_density, dim3((unsigned)nbr), dim3(256), 0, s, M->m,
+                           (int)nbc, M->csr_rowptr, M->csr_colidx, cnt + 1, ints + 4);
         SWIFT_CHECK_LAUNCH();
     }
     if (m) {
@@ -116,11 +170,14 @@ int matrix_stats_impl(const swift_matrix *M, swift_stats *o, hipStream_t s) {
     unsigned char h[64];
     SWIFT_HIP(hipMemcpyAsync(h, buf.ptr, 64, hipMemcpyDeviceToHost, s));
     SWIFT_HIP(hipStreamSynchronize(s));
-    unsigned long long ne;
-    int hi[4];
-    std::memcpy(&ne, h, 8);
-    std::memcpy(hi, h + 8, 16);
-    o->nonempty_blocks = (long long)ne;
+    unsigned long long ne[3];
+    int hi[5];
+    std::memcpy(ne, h, 24);
+    std::memcpy(hi, h + 24, 20);
+    o->nonempty_blocks = (long long)ne[0];
+    o->dense_blocks = (long long)ne[1];
+    o->dense_block_nnz = (long long)ne[2];
+    o->max_block_nnz = hi[4];
     o->empty_block_ratio = o->blocks ? (double)(o->blocks - o->nonempty_blocks) / o->blocks : 0.0;
     o->max_row_nnz = hi[0];
     for (int k = 0; k < 3; k++) o->tile_format_count[k] = hi[1 + k];

@@ -529,6 +529,10 @@ static int mode_mtxinfo(const Args &a, Problem<T> &P) {
            st.sparsity, st.empty_block_ratio, st.nonempty_blocks, st.blocks, st.regular_nnz,
            st.regular_ratio, st.reside_nnz, st.tile_format_count[0], st.tile_format_count[1],
            st.tile_format_count[2], st.max_row_nnz, st.max_col_nnz);
+    printf("dense blocks (>= 512 of 1024 entries) %lld holding %lld entries (%.4f of nnz); "
+           "fullest block %d entries\n",
+           st.dense_blocks, st.dense_block_nnz,
+           P.nnz ? (double)st.dense_block_nnz / P.nnz : 0.0, st.max_block_nnz);
     Operands<T> o;
     make_operands(P, a.K, o);
     const float tv = vendor_time(P, o, a.reps);

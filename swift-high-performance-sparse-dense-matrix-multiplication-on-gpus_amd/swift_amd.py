@@ -54,7 +54,9 @@ class Stats(C.Structure):
                 ("regular_nnz", C.c_int), ("reside_nnz", C.c_int),
                 ("regular_ratio", C.c_double), ("tilenum", C.c_int),
                 ("tile_format_count", C.c_int * 3), ("max_row_nnz", C.c_int),
-                ("max_col_nnz", C.c_int), ("mean_row_nnz", C.c_double)]
+                ("max_col_nnz", C.c_int), ("mean_row_nnz", C.c_double),
+                ("dense_blocks", C.c_longlong), ("dense_block_nnz", C.c_longlong),
+                ("max_block_nnz", C.c_int)]
 
 
 # every symbol include/swift_spmm.h declares

@@ -30,6 +30,9 @@ def check(swift, oracle, m, n, rp, ci, v):
     assert st.max_row_nnz == (int(lens.max()) if m else 0)
     colcnt = np.bincount(np.asarray(ci, np.int64), minlength=n) if len(ci) else np.zeros(1, int)
     assert st.max_col_nnz == int(colcnt.max())
+    dense, dense_nnz, max_block = oracle.block_density(m, n, rp, ci, 512)
+    assert (st.dense_blocks, st.dense_block_nnz, st.max_block_nnz) == (dense, dense_nnz,
+                                                                        max_block)
     return st
 
 
@@ -47,6 +50,22 @@ def test_stats_large(swift, oracle, cuda):
     assert 0.0 < st.empty_block_ratio < 1.0
 
 
+def test_stats_dense_blocks(swift, oracle, cuda):
+    """Blocks at least half full: a banded matrix (bandwidth 12, every band
+    block dense) and a random one around a 60%-filled 64 x 64 corner."""
+    import matgen
+    st = check(swift, oracle, *matgen.banded_mixed_csr(2000, 2000, 5, bw=12, lo=25, hi=25,
+                                                      mixed=False))
+    assert st.dense_blocks > 0 and st.max_block_nnz >= 512
+    rng = np.random.default_rng(9)
+    A = rng.random((300, 400)) < 0.01
+    A[:64, :64] |= rng.random((64, 64)) < 0.6
+    rp = np.r_[0, np.cumsum(A.sum(1))].astype(np.int32)
+    ci = np.concatenate([np.nonzero(r)[0] for r in A]).astype(np.int32)
+    st = check(swift, oracle, 300, 400, rp, ci, np.ones(len(ci)))
+    assert st.dense_blocks >= 4
+
+
 def test_stats_wide_bitmap(swift, oracle, cuda):
     """More column blocks than one bitmap pass holds (262144): 64 x 9,000,000."""
     import torch
@@ -61,3 +80,4 @@ def test_stats_wide_bitmap(swift, oracle, cuda):
     blocks = 2 * (n // 32 + (n % 32 > 0))
     occupied = len(set((i // 32, int(c) // 32) for i in range(m) for c in rows[i]))
     assert st.blocks == blocks and st.nonempty_blocks == occupied
+    assert (st.dense_blocks, st.dense_block_nnz) == (0, 0) and st.max_block_nnz >= 1

@@ -237,3 +237,32 @@ def test_block_stats_known_blocks(oracle):
     ci = np.array([0, 95, 40], np.int32)   # row 5: blocks (0,0), (0,2); row 39: block (1,1)
     blocks, empty, _ = oracle.block_stats(64, 96, rp, ci)
     assert (blocks, empty) == (6, 3)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (64, 64), (100, 257), (97, 130)])
+def test_block_density_brute_force(oracle, shape):
+    """Dense 32x32 blocks against a dense block reduction: a random pattern plus
+    one block filled to 60% and one to exactly the threshold."""
+    m, n = shape
+    rng = np.random.default_rng(m * 1000 + n)
+    A = rng.random((m, n)) < 0.02
+    if m >= 64 and n >= 64:
+        A[0:32, 32:64] = rng.random((32, 32)) < 0.6
+        blk = np.zeros(1024, bool)
+        blk[:512] = True
+        A[32:64, 0:32] = rng.permutation(blk).reshape(32, 32)
+    rp = np.r_[0, np.cumsum(A.sum(1))].astype(np.int32)
+    ci = np.concatenate([np.nonzero(r)[0] for r in A]).astype(np.int32) if A.any() else \
+        np.zeros(0, np.int32)
+    rb, cb = -(-m // 32), -(-n // 32)
+    P = np.zeros((rb * 32, cb * 32), int)
+    P[:m, :n] = A
+    cnt = P.reshape(rb, 32, cb, 32).sum(axis=(1, 3))
+    for thr in (1, 300, 512):
+        d, dz, mx = oracle.block_density(m, n, rp, ci, thr)
+        assert d == int((cnt >= thr).sum())
+        assert dz == int(cnt[cnt >= thr].sum())
+        assert mx == int(cnt.max())
+    if m >= 64 and n >= 64:
+        d, dz, _ = oracle.block_density(m, n, rp, ci, 512)
+        assert d >= 2 and dz >= 2 * 512
