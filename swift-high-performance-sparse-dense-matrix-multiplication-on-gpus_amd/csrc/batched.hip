@@ -3,10 +3,11 @@
 //
 // A batch is a list of built matrices (FP32, row-major, K in {32, 64, 128,
 // 256} per matrix).  Their row-view segments are flattened into one work list
-// of (matrix, segment) items; a persistent kernel sized to the device (32
-// waves per CU) hands items to waves through an atomic counter, and each wave
-// runs the same walk as the single-matrix row-view kernel with the matrix's
-// own lane shape (chosen by a wave-uniform switch on K).  Split rows are
+// of (matrix, segment) items, cut into 8 contiguous ranges of about equal
+// work, one per XCD; a persistent kernel sized to the device (32 waves per
+// CU) hands items to waves through one atomic counter per range, and each
+// wave runs the same walk as the single-matrix row-view kernel with the
+// matrix's own lane shape (chosen by a wave-uniform switch on K).  Split rows are
 // zeroed first and the reside blocks of all matrices follow in one more
 // kernel.  swift_batch_graph_launch captures the four launches (counter reset,
 // split-row zeroing, persistent walk, reside blocks) into a HIP graph once and
@@ -34,7 +35,11 @@ int spmm_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dty
 
 // ------------------------------------------------------------ kernels --
 
-__global__ void k_batch_reset(int *counter) { *counter = 0; }
+constexpr int NXCD = 8;  // XCDs of an MI355X; blocks are dealt to them round-robin
+
+__global__ void k_batch_reset(int *counter) {
+    if (threadIdx.x < NXCD) counter[threadIdx.x] = 0;
+}
 
 // (matrix, row) pairs: zero the row of that matrix's C.
 __global__ __launch_bounds__(256) void k_batch_zero(const RowArgs *__restrict__ desc,
@@ -52,17 +57,41 @@ __device__ __forceinline__ void batch_walk(const RowArgs &a, int sid) {
     rowview_walk<float, float, float, 4, 1, LPE, false, BUF>(a, sid);
 }
 
-// Persistent: every wave takes items until the counter passes nitems (an exit
-// every wave reaches).  BUF: every matrix's B fits the buffer-resource path.
+// Next item of a wave: items range[x] .. range[x+1]-1 belong to XCD x and
+// counter[x] counts the ones handed out.  A wave drains its own XCD's range
+// first (block b runs on XCD b % 8, so the waves of one XCD walk neighbouring
+// segments of the same matrices and their B rows meet in that XCD's L2), then
+// the other ranges in turn, so the tail stays balanced.  Returns -1 once all
+// NXCD ranges are exhausted: counters only grow, so every wave reaches the
+// exit after at most NXCD empty claims.
+__device__ __forceinline__ int batch_claim(const int *__restrict__ range,
+                                           int *__restrict__ counter, int &x, int &left) {
+    while (left > 0) {
+        int it = 0;
+        if ((threadIdx.x & 63) == 0) it = atomicAdd(counter + x, 1);
+        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64)) + range[x];
+        if (it < range[x + 1]) return it;
+        x = x + 1 == NXCD ? 0 : x + 1;
+        left--;
+    }
+    return -1;
+}
+
+// Persistent: every wave claims items until every range is exhausted (an
+// exit every wave reaches); the next item is claimed before the current one
+// is walked, so the atomic's round trip overlaps the walk.  BUF: every
+// matrix's B fits the buffer-resource path.
 template <bool BUF>
 __global__ __launch_bounds__(256) void k_batch_walk(const RowArgs *__restrict__ desc,
-                                                    const int2 *__restrict__ items, int nitems,
+                                                    const int2 *__restrict__ items,
+                                                    const int *__restrict__ range,
                                                     int *__restrict__ counter) {
+    int x = (int)(blockIdx.x % NXCD), left = NXCD;
+    int next = batch_claim(range, counter, x, left);
     for (;;) {
-        int it = 0;
-        if ((threadIdx.x & 63) == 0) it = atomicAdd(counter, 1);
-        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
-        if (it >= nitems) break;
+        const int it = next;
+        if (it < 0) break;
+        next = batch_claim(range, counter, x, left);
         const int2 w = items[it];
         const int mi = __builtin_amdgcn_readfirstlane(w.x);
         const int sid = __builtin_amdgcn_readfirstlane(w.y);
@@ -157,7 +186,8 @@ struct swift_batch {
     RowArgs *desc = nullptr;
     int2 *items = nullptr, *res_items = nullptr, *split = nullptr;
     int nitems = 0, nres = 0, nsplit = 0;
-    int *counter = nullptr;
+    int *counter = nullptr;  // NXCD claim counters
+    int *range = nullptr;    // NXCD+1 item bounds of the per-XCD ranges
     int grid = 0;
     bool buf = true;  // all B operands fit the buffer-resource path
     std::vector<const float *> lastB;
@@ -172,7 +202,7 @@ static void batch_free(swift_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void *p : {(void *)b->desc, (void *)b->items, (void *)b->res_items, (void *)b->split,
-                    (void *)b->counter})
+                    (void *)b->counter, (void *)b->range})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -272,9 +302,15 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
             // flattened work lists (host) -> device
             std::vector<int2> items, res, split;
             std::vector<int> rows;
+            std::vector<double> work;  // prefix of segment items x K per item
+            double total = 0.0;
             for (int i = 0; i < count; i++) {
                 swift_matrix *M = mats[i];
-                for (int q = 0; q < M->nseg; q++) items.push_back(make_int2(i, q));
+                for (int q = 0; q < M->nseg; q++) {
+                    items.push_back(make_int2(i, q));
+                    total += (double)b->W[i] * b->Ks[i];
+                    work.push_back(total);
+                }
                 for (int q = 0; q < M->numblk; q++) res.push_back(make_int2(i, q));
                 rows.resize((size_t)M->nsplit);
                 if (M->nsplit &&
@@ -304,7 +340,20 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
             if (rc == SWIFT_OK) rc = up((void **)&b->split, split.data(), split.size() * sizeof(int2));
             if (rc == SWIFT_OK)
                 rc = up((void **)&b->desc, b->desc_h.data(), b->desc_h.size() * sizeof(RowArgs));
-            if (rc == SWIFT_OK) rc = up((void **)&b->counter, nullptr, 0);
+            const int zeros[NXCD] = {};
+            if (rc == SWIFT_OK) rc = up((void **)&b->counter, zeros, sizeof(zeros));
+            // range x ends at the first item whose work prefix reaches
+            // (x+1)/NXCD of the total
+            int bounds[NXCD + 1];
+            bounds[0] = 0;
+            for (int x = 1; x <= NXCD; x++) {
+                const double goal = total * x / NXCD;
+                bounds[x] = x == NXCD ? (int)items.size()
+                                      : (int)(std::lower_bound(work.begin(), work.end(), goal) -
+                                              work.begin());
+                bounds[x] = std::max(bounds[x], bounds[x - 1]);
+            }
+            if (rc == SWIFT_OK) rc = up((void **)&b->range, bounds, sizeof(bounds));
             b->nitems = (int)items.size();
             b->nres = (int)res.size();
             b->nsplit = (int)split.size();
@@ -367,17 +416,17 @@ static int batch_set_operands(swift_batch *b, const float *const *dB, float *con
 }
 
 static int batch_launch(swift_batch *b, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, b->counter);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->counter);
     if (b->nsplit)
         hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div((size_t)b->nsplit * 64, 256)), dim3(256), 0,
                            s, b->desc, b->split, b->nsplit);
     if (b->nitems) {
         if (b->buf)
             hipLaunchKernelGGL(k_batch_walk<true>, dim3(b->grid), dim3(256), 0, s, b->desc,
-                               b->items, b->nitems, b->counter);
+                               b->items, b->range, b->counter);
         else
             hipLaunchKernelGGL(k_batch_walk<false>, dim3(b->grid), dim3(256), 0, s, b->desc,
-                               b->items, b->nitems, b->counter);
+                               b->items, b->range, b->counter);
     }
     if (b->nres)
         hipLaunchKernelGGL(k_batch_reside, dim3(ceil_div((size_t)b->nres * 64, 256)), dim3(256), 0,
