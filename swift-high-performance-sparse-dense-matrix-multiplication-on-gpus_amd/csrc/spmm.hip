@@ -640,9 +640,72 @@ __global__ __launch_bounds__(256) void k_transpose(const T *__restrict__ in, T *
     }
 }
 
+// 16-byte form: tiles of 32 rows x 32*V columns (V elements per 16 bytes,
+// 16 KiB per tile), read as 4 row vectors and written as 4 column vectors per
+// thread, so each block keeps 16 KiB of loads in flight instead of 4 KiB.
+// Needs rows and cols multiples of V and 16-byte aligned bases.
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose16(const T *__restrict__ in, T *__restrict__ out,
+                                                     int rows, int cols, int tiles_c,
+                                                     int accumulate) {
+    constexpr int V = 16 / (int)sizeof(T);
+    constexpr int TC = 32 * V;           // tile columns
+    constexpr int RG = 32 / V;           // row vectors per output row of the tile
+    __shared__ T tile[32][TC + 1];
+    const int tcol = (int)(blockIdx.x % (unsigned)tiles_c), trow = (int)(blockIdx.x / (unsigned)tiles_c);
+    const int c0 = tcol * TC, r0 = trow * 32;
+    const int t = threadIdx.x;
+    // 32 rows x 32 vectors: vector t + 256*i is (row (t>>5) + 8i, column vector t & 31)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int rr = (t >> 5) + 8 * i, cv = t & 31;
+        const int r = r0 + rr, c = c0 + cv * V;
+        if (r < rows && c < cols) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(in + (size_t)r * cols + c);
+            const T *e = reinterpret_cast<const T *>(&v);
+#pragma unroll
+            for (int k = 0; k < V; k++) tile[rr][cv * V + k] = e[k];
+        }
+    }
+    __syncthreads();
+    // TC output rows x RG vectors: vector t + 256*i is (output row
+    // (t + 256i) / RG, rows rg*V .. rg*V + V-1 of the tile)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int q = t + 256 * i;
+        const int cc = q / RG, rg = q % RG;
+        const int c = c0 + cc, r = r0 + rg * V;
+        if (c < cols && r < rows) {
+            T x[V];
+#pragma unroll
+            for (int k = 0; k < V; k++) x[k] = tile[rg * V + k][cc];
+            T *o = out + (size_t)c * rows + r;
+            if (accumulate) {
+                const uint4 old = *reinterpret_cast<const uint4 *>(o);
+                const T *e = reinterpret_cast<const T *>(&old);
+#pragma unroll
+                for (int k = 0; k < V; k++) x[k] = (T)(x[k] + e[k]);
+            }
+            uint4 w;
+            __builtin_memcpy(&w, x, 16);
+            *reinterpret_cast<uint4 *>(o) = w;
+        }
+    }
+}
+
 template <class T>
 static int transpose(const T *in, T *out, int rows, int cols, bool accumulate, hipStream_t s) {
     if (rows == 0 || cols == 0) return SWIFT_OK;
+    constexpr int V = 16 / (int)sizeof(T);
+    if (rows % V == 0 && cols % V == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0) {
+        const size_t tc = ceil_div(cols, 32 * V), tr = ceil_div(rows, 32);
+        if (tc * tr <= 0x7FFFFFFFull) {
+            hipLaunchKernelGGL(k_transpose16<T>, dim3((unsigned)(tc * tr)), dim3(256), 0, s, in,
+                               out, rows, cols, (int)tc, accumulate ? 1 : 0);
+            SWIFT_CHECK_LAUNCH();
+            return SWIFT_OK;
+        }
+    }
     const size_t tc = ceil_div(cols, 32), tr = ceil_div(rows, 32);
     if (tc * tr > 0x7FFFFFFFull) {
         set_error("transpose: %d x %d is too large", rows, cols);
