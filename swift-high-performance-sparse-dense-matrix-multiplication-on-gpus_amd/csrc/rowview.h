@@ -13,12 +13,16 @@ namespace swift {
 
 // Elements per lane per B load: 16-byte loads for FP32/FP64; 8-byte loads of
 // four 16-bit values, which keeps the lane shape (and so the per-entry
-// instruction count) of FP32 while moving half the bytes.
+// instruction count) of FP32 while moving half the bytes (SWIFT_NARROW_VEC=8
+// builds 16-byte loads of eight).
+#ifndef SWIFT_NARROW_VEC
+#define SWIFT_NARROW_VEC 4
+#endif
 template <class T> struct Vec16;
 template <> struct Vec16<float> { static constexpr int N = 4; };
 template <> struct Vec16<double> { static constexpr int N = 2; };
-template <> struct Vec16<__hip_bfloat16> { static constexpr int N = 4; };
-template <> struct Vec16<__half> { static constexpr int N = 4; };
+template <> struct Vec16<__hip_bfloat16> { static constexpr int N = SWIFT_NARROW_VEC; };
+template <> struct Vec16<__half> { static constexpr int N = SWIFT_NARROW_VEC; };
 
 __device__ __forceinline__ float to_acc(float x) { return x; }
 __device__ __forceinline__ double to_acc(double x) { return x; }
