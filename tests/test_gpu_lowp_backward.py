@@ -182,3 +182,96 @@ def test_autograd_finite_differences(swift, cuda, fused):
         fd = (f(v, Bp) - f(v, Bm)) / (2 * h)
         assert abs(fd - gB[i, k]) <= 1e-6 * max(1.0, abs(fd))
     M.close()
+
+
+def _gnn_graph(swift):
+    d = swift.generate(4, scale=1.0, seed=0)  # 200,000 nodes, ~10.2M edges
+    return d["m"], d["rowptr"].astype(np.int64), d["colidx"].astype(np.int64), d["val"]
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_gnn_induced_subgraph_gradients(swift, cuda, fused):
+    """The 2,000-node subgraph induced by the highest-degree nodes of the
+    200k-node graph, with the full graph's D^-1/2 (A+I) D^-1/2 values: the
+    autograd wrapper's gradients of loss = <W, A(vals) B> against dense FP64
+    torch autograd on the CPU, then against FP64 central differences."""
+    import torch
+    n_all, rp, ci, v = _gnn_graph(swift)
+    deg = np.diff(rp)
+    nodes = np.sort(np.argsort(-deg, kind="stable")[:2000])
+    new = np.full(n_all, -1, np.int64)
+    new[nodes] = np.arange(len(nodes))
+    row = np.repeat(np.arange(n_all), deg)
+    keep = (new[row] >= 0) & (new[ci] >= 0)
+    r2, c2, v2 = new[row[keep]], new[ci[keep]], v[keep]
+    n = len(nodes)
+    rp2 = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(r2, minlength=n), out=rp2[1:])
+    assert len(v2) > 20 * n  # the hubs are densely connected among themselves
+    K = 16
+    rng = np.random.default_rng(0)
+    B0 = rng.standard_normal((n, K))
+    W0 = rng.standard_normal((n, K))
+    M = swift.SwiftMatrix(n, n, rp2.astype(np.int32), c2.astype(np.int32), v2,
+                          dtype=torch.float64)
+    Fn = swift.spmm_function()
+    vals = torch.tensor(v2, device="cuda", requires_grad=True)
+    B = torch.tensor(B0, device="cuda", requires_grad=True)
+    W = torch.tensor(W0, device="cuda")
+    (Fn.apply(vals, B, M, fused) * W).sum().backward()
+    gv, gB = vals.grad.cpu().numpy(), B.grad.cpu().numpy()
+    # dense reference
+    rv = torch.tensor(v2, dtype=torch.float64, requires_grad=True)
+    rB = torch.tensor(B0, requires_grad=True)
+    A = torch.zeros((n, n), dtype=torch.float64).index_put(
+        (torch.tensor(r2), torch.tensor(c2)), rv, accumulate=True)
+    (torch.mm(A, rB) * torch.tensor(W0)).sum().backward()
+    np.testing.assert_allclose(gv, rv.grad.numpy(), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(gB, rB.grad.numpy(), rtol=1e-12, atol=1e-12)
+
+    def f(vv, BB):
+        M.update_values(torch.tensor(vv, device="cuda"))
+        return float((M.spmm(torch.tensor(BB, device="cuda")) * W).sum())
+
+    h = 1e-6
+    for e in rng.choice(len(v2), 6, replace=False):
+        vp, vm = v2.copy(), v2.copy()
+        vp[e] += h
+        vm[e] -= h
+        fd = (f(vp, B0) - f(vm, B0)) / (2 * h)
+        assert abs(fd - gv[e]) <= 1e-6 * max(1.0, abs(fd))
+    M.close()
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("fused", [True, False])
+def test_gnn_full_adjoint_identity(swift, cuda, fused):
+    """At the full 200k-node size (FP32): <dC, A B> = <A^T dC, B> =
+    sum_e vals_e dvals_e, summed in FP64.  The three agree to ~1.4e-11 of
+    S = <|dC|, |A| |B|> (~1e7 here) while the sums themselves are ~170 (random
+    signs cancel); the bound 1e-9 S (~0.01) is tight enough that one dropped
+    or doubled edge (~0.2) fails it."""
+    import torch
+    n, rp, ci, v = _gnn_graph(swift)
+    K = 128
+    M = swift.SwiftMatrix(n, n, rp.astype(np.int32), ci.astype(np.int32),
+                          v.astype(np.float32), dtype=torch.float32)
+    Mabs = swift.SwiftMatrix(n, n, rp.astype(np.int32), ci.astype(np.int32),
+                             np.abs(v).astype(np.float32), dtype=torch.float32)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    B = torch.randn((n, K), device="cuda", generator=g)
+    dC = torch.randn((n, K), device="cuda", generator=g)
+    C = M.spmm(B)
+    if fused:
+        dB, dv = M.backward_fused(dC, B)
+    else:
+        dB, dv = M.transpose().spmm(dC), M.sddmm(dC, B)
+    s1 = float((dC.double() * C.double()).sum())
+    s2 = float((dB.double() * B.double()).sum())
+    s3 = float((torch.tensor(v, device="cuda") * dv.double()).sum())
+    scale = float((dC.abs().double() * Mabs.spmm(B.abs()).double()).sum())
+    print("adjoint", s1, s2, s3, "scale", scale, flush=True)
+    assert abs(s1 - s2) <= 1e-9 * scale
+    assert abs(s1 - s3) <= 1e-9 * scale
+    Mabs.close()
+    M.close()
