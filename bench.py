@@ -94,6 +94,32 @@ def cpu_baseline(d, K, seed):
                       % (m, n, nnz, K, ms)}
 
 
+def vendor_baseline(scale, K, nnz):
+    """rocSPARSE rocsparse_spmm (CSR, default algorithm) on the same generated
+    matrix, through the C driver's vendor mode (a child process): row-major
+    and column-major B/C, each checked against the host golden."""
+    import subprocess
+    exe = os.path.join(PKG, "swift_driver")
+    cmd = [exe, "--gen", "2", "--scale", repr(scale), "--fp32", "--seed", "0", "--reps", "10",
+           "--mode", "vendor", str(K)]
+    try:
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=600).stdout
+    except (OSError, subprocess.TimeoutExpired) as e:
+        return {"value": None, "error": str(e)}
+    for line in out.splitlines():
+        f = line.split()
+        if f[:2] == ["vendor", "rocSPARSE"]:
+            row, col = float(f[f.index("row-major") + 1]), float(f[f.index("col-major") + 1])
+            ok = [t for t in (row, col) if t > 0]
+            best = min(ok) if ok else None
+            return {"value": round(2.0 * nnz * K / (best * 1e6), 2) if best else None,
+                    "unit": "GFLOP/s", "name": "rocSPARSE rocsparse_spmm (CSR, alg default)",
+                    "row_major_ms": row, "col_major_ms": col,
+                    "sample": "same matrix and K, B in [-0.5, 0.5), HIP events around the "
+                              "compute stage, median of 10"}
+    return {"value": None, "error": out[-300:]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +128,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the matrix (tests only)")
     ap.add_argument("--K", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE baseline")
     ap.add_argument("--extras", action="store_true", help="also time the other forms")
     ap.add_argument("--batch", type=int, default=0,
                     help="also time the batched form over this many generated matrices")
@@ -222,6 +249,10 @@ def main():
     if args.batch:
         out["batched"] = batched(S, torch, args.batch, args.scale)
         torch.cuda.empty_cache()
+    if not args.no_vendor and args.part == "all":
+        t0 = time.time()
+        out["vendor_baseline"] = vendor_baseline(args.scale, K, nnz)
+        log("vendor baseline %.1fs: %s" % (time.time() - t0, out["vendor_baseline"]))
     if not args.no_cpu:
         t0 = time.time()
         out["cpu_baseline"] = cpu_baseline(d, K, seed)

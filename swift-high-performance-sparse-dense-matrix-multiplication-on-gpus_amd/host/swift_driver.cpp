@@ -26,6 +26,8 @@
 //               (test/k-32/irr_opt): "... non-opt T opt T"
 //   preprocess  preprocessing cost (test/preprocess/main.cu:440-456):
 //               "... sort T ratio R block T ratio R preprocess T final T overall_time T"
+//   vendor      rocSPARSE SpMM alone, column- and row-major (the vendor
+//               baseline of bench.py)
 //   mtxinfo     matrix statistics (test/MTX_info/main.cu:484-493 and
 //               test/irr_regular_ratio/main.cu:216): "... rocSPARSE: T
 //               sparsity S emptyblkratio E final T" and "<name> nnz NNZ
@@ -110,9 +112,10 @@ static int verify(const T *got, const T *golden, size_t n) {
 
 // rocSPARSE CSR SpMM with column-major B/C (the reference's cuSPARSE call,
 // src/gpu_cusparse_spmm.h:75-95); returns the median kernel time in ms.
+// row_major: B (n x K) and C (m x K) row-major instead (rocsparse_order_row).
 template <class T>
 static float vendor_spmm(int m, int n, int nnz, int K, const int *rp, const int *ci,
-                         const T *val, const T *Bcol, T *Ccol, int reps) {
+                         const T *val, const T *Bcol, T *Ccol, int reps, bool row_major = false) {
     int *drp, *dci;
     T *dv, *dB, *dC;
     HIP_OK(hipMalloc(&drp, sizeof(int) * (m + 1)));
@@ -131,8 +134,13 @@ static float vendor_spmm(int m, int n, int nnz, int K, const int *rp, const int 
     RS_OK(rocsparse_create_csr_descr(&A, m, n, nnz, drp, dci, dv, rocsparse_indextype_i32,
                                      rocsparse_indextype_i32, rocsparse_index_base_zero,
                                      Traits<T>::rt));
-    RS_OK(rocsparse_create_dnmat_descr(&Bd, n, K, n, dB, Traits<T>::rt, rocsparse_order_column));
-    RS_OK(rocsparse_create_dnmat_descr(&Cd, m, K, m, dC, Traits<T>::rt, rocsparse_order_column));
+    if (row_major) {
+        RS_OK(rocsparse_create_dnmat_descr(&Bd, n, K, K, dB, Traits<T>::rt, rocsparse_order_row));
+        RS_OK(rocsparse_create_dnmat_descr(&Cd, m, K, K, dC, Traits<T>::rt, rocsparse_order_row));
+    } else {
+        RS_OK(rocsparse_create_dnmat_descr(&Bd, n, K, n, dB, Traits<T>::rt, rocsparse_order_column));
+        RS_OK(rocsparse_create_dnmat_descr(&Cd, m, K, m, dC, Traits<T>::rt, rocsparse_order_column));
+    }
     T alpha = 1, beta = 0;
     size_t bytes = 0;
     RS_OK(rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, Bd,
@@ -300,6 +308,22 @@ static float vendor_time(const Problem<T> &P, const Operands<T> &o, int reps) {
         return -1111;
     }
     printf("success rocSPARSE SpMM, process:%f ms\n", t);
+    return t;
+}
+
+// rocSPARSE with row-major B and C (the layout of the row-major Swift
+// default) with ResultVerify; -1111 when it disagrees.
+template <class T>
+static float vendor_time_row(const Problem<T> &P, const Operands<T> &o, int reps) {
+    std::vector<T> C((size_t)P.m * o.K);
+    float t = vendor_spmm<T>(P.m, P.n, P.nnz, o.K, P.rp, P.ci, P.val.data(), o.B.data(), C.data(),
+                             reps, true);
+    int bad = verify(C.data(), o.golden.data(), C.size());
+    if (bad) {
+        printf("error rocSPARSE SpMM (row-major), error = %d\n", bad);
+        return -1111;
+    }
+    printf("success rocSPARSE SpMM (row-major), process:%f ms\n", t);
     return t;
 }
 
@@ -514,6 +538,21 @@ static int mode_preprocess(const Args &a, Problem<T> &P) {
     return 0;
 }
 
+// vendor: rocSPARSE SpMM alone, column-major (the reference's cuSPARSE
+// call, src/gpu_cusparse_spmm.h:75-95) and row-major, each verified against
+// the host golden; one line on stdout:
+// "vendor rocSPARSE m M n N width K nnz NNZ row-major T col-major T".
+template <class T>
+static int mode_vendor(const Args &a, Problem<T> &P) {
+    Operands<T> o;
+    make_operands(P, a.K, o);
+    const float tc = vendor_time(P, o, a.reps);
+    const float tr = vendor_time_row(P, o, a.reps);
+    printf("vendor rocSPARSE m %d n %d width %d nnz %d row-major %f col-major %f\n", P.m, P.n,
+           a.K, P.nnz, tr, tc);
+    return (tc < 0 || tr < 0) ? 2 : 0;
+}
+
 // mtxinfo: sparsity and the share of empty 32x32 blocks with the product
 // times (test/MTX_info/main.cu:484-493), and the sizes of Swift's regular
 // and irregular parts (test/irr_regular_ratio/main.cu:216).
@@ -565,6 +604,7 @@ static int run(const Args &a) {
     if (a.mode == "irropt") return mode_irropt(a, P);
     if (a.mode == "preprocess") return mode_preprocess(a, P);
     if (a.mode == "mtxinfo") return mode_mtxinfo(a, P);
+    if (a.mode == "vendor") return mode_vendor(a, P);
     fprintf(stderr, "unknown --mode %s\n", a.mode.c_str());
     return 2;
 }

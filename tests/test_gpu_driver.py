@@ -87,3 +87,14 @@ def test_mtxinfo_mode(tmp_path):
     assert r[3] == "regulatpart" and r[6] == "irregularpart"
     assert int(r[4]) + int(r[7]) == int(r[2])
     assert float(r[5]) + float(r[8]) == pytest.approx(1.0, abs=1e-5)
+
+
+@pytest.mark.parametrize("prec", ["--fp32", "--fp64"])
+def test_vendor_mode(tmp_path, prec):
+    """rocSPARSE alone, row- and column-major, both verified against the host
+    golden (the vendor leg of bench.py)."""
+    out = run(["--gen", "1", "--scale", "0.25", prec, "--mode", "vendor", "32"], tmp_path)
+    (v,) = [l.split() for l in out.splitlines() if l.startswith("vendor rocSPARSE")]
+    assert v[10] == "row-major" and v[12] == "col-major"
+    assert float(v[11]) > 0 and float(v[13]) > 0
+    assert "success rocSPARSE SpMM (row-major)" in out and "success rocSPARSE SpMM," in out
