@@ -115,6 +115,16 @@ int swift_mmio_allinone(int *m, int *n, int *nnz, int *isSymmetric,
 int swift_mmio_allinone_f32(int *m, int *n, int *nnz, int *isSymmetric,
                             int **csrrowptr, int **csrcolidx, float **csrval,
                             const char *filename);
+/* The same loader with the value type as an argument and an explicit parser
+ * thread count: 1 = the sequential fscanf reading of the reference; n > 1 =
+ * n threads over the mapped file (identical arrays; a token the fast
+ * conversion does not take whole falls back to the sequential reading);
+ * 0 = automatic (up to 16 threads for files of 1 MiB or more).  No
+ * counterpart in the reference, whose loader reads entry by entry with
+ * fscanf (src/auxiliary.h:238-276). */
+int swift_mmio_allinone_ex(int *m, int *n, int *nnz, int *isSymmetric,
+                           int **csrrowptr, int **csrcolidx, void **csrval,
+                           swift_dtype vtype, const char *filename, int nthreads);
 void swift_host_free(void *p);
 
 /* Seeded synthetic inputs of the benchmark configurations (host memory,

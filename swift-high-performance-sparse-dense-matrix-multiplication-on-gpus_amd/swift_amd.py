@@ -62,7 +62,7 @@ class Stats(C.Structure):
 # every symbol include/swift_spmm.h declares
 EXPORTS = (
     "swift_last_error", "swift_version", "swift_mmio_allinone", "swift_mmio_allinone_f32",
-    "swift_host_free", "swift_generate", "swift_generate_dense", "swift_csr_to_csc",
+    "swift_mmio_allinone_ex", "swift_host_free", "swift_generate", "swift_generate_dense", "swift_csr_to_csc",
     "swift_col_sort", "swift_build", "swift_build_host", "swift_destroy",
     "swift_row_idx_sort", "swift_set_segment_items", "swift_get_info", "swift_matrix_stats",
     "swift_permute_dense", "swift_spmm",
@@ -127,6 +127,9 @@ def lib():
         L.swift_update_values.argtypes = [_vp, _vp, _vp]
         L.swift_mmio_allinone.argtypes = [_P(C.c_int)] * 4 + [_P(_P(C.c_int)), _P(_P(C.c_int)),
                                                                 _P(_P(C.c_double)), C.c_char_p]
+        L.swift_mmio_allinone_ex.argtypes = [_P(C.c_int)] * 4 + [_P(_P(C.c_int)),
+                                                                   _P(_P(C.c_int)), _P(_vp),
+                                                                   C.c_int, C.c_char_p, C.c_int]
         L.swift_batch_create.argtypes = [_P(_vp), _vp, _vp, C.c_int, _vp]
         L.swift_batch_spmm.argtypes = [_vp, _vp, _vp, _vp]
         L.swift_batch_graph_launch.argtypes = [_vp, _vp, _vp, _vp]
@@ -180,14 +183,29 @@ def generate_dense(count, dist=0, lo=-1.0, hi=1.0, seed=0):
     return out
 
 
-def mmio_load(path):
-    """swift_mmio_allinone -> (status, dict or None)."""
+def mmio_load(path, threads=None, dtype=F64):
+    """swift_mmio_allinone (threads None) or swift_mmio_allinone_ex with that
+    parser thread count (0 automatic, 1 sequential) -> (status, dict or None)."""
     m, n, nnz, sym = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-    rp, ci, vv = _P(C.c_int)(), _P(C.c_int)(), _P(C.c_double)()
-    rc = lib().swift_mmio_allinone(C.byref(m), C.byref(n), C.byref(nnz), C.byref(sym),
-                                   C.byref(rp), C.byref(ci), C.byref(vv), path.encode())
+    rp, ci = _P(C.c_int)(), _P(C.c_int)()
+    if threads is None:
+        vv = _P(C.c_double)()
+        rc = lib().swift_mmio_allinone(C.byref(m), C.byref(n), C.byref(nnz), C.byref(sym),
+                                       C.byref(rp), C.byref(ci), C.byref(vv), path.encode())
+    else:
+        raw = _vp()
+        rc = lib().swift_mmio_allinone_ex(C.byref(m), C.byref(n), C.byref(nnz), C.byref(sym),
+                                          C.byref(rp), C.byref(ci), C.byref(raw), dtype,
+                                          path.encode(), threads)
+        if rc == 0:
+            vv = C.cast(raw, _P(C.c_double if dtype == F64 else C.c_float))
     if rc != 0:
         return rc, None
+    if threads is not None and dtype == F32:
+        return 0, dict(m=m.value, n=n.value, nnz=nnz.value, sym=sym.value,
+                       rowptr=_take(rp, m.value + 1, np.int32),
+                       colidx=_take(ci, nnz.value, np.int32),
+                       val=_take(vv, nnz.value, np.float32))
     return 0, dict(m=m.value, n=n.value, nnz=nnz.value, sym=sym.value,
                    rowptr=_take(rp, m.value + 1, np.int32),
                    colidx=_take(ci, nnz.value, np.int32),

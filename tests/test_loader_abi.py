@@ -114,3 +114,91 @@ def test_dense_generator(swift):
     assert x.min() >= -1 and x.max() < 1 and abs(x.mean()) < 0.05
     y = swift.generate_dense(10000, 1, seed=3)
     assert abs(y.std() - 1) < 0.05
+
+
+def _same(a, b):
+    """Loader results equal bit for bit (values compared as bytes: NaN too)."""
+    (ra, da), (rb, db) = a, b
+    if ra != rb:
+        return False
+    if ra != 0:
+        return True
+    return all(da[k] == db[k] for k in ("m", "n", "nnz", "sym")) and \
+        all(da[k].tobytes() == db[k].tobytes() for k in ("rowptr", "colidx", "val"))
+
+
+@pytest.mark.parametrize("threads", [2, 3, 8])
+@pytest.mark.parametrize("path", MTX, ids=[os.path.basename(p) for p in MTX])
+def test_parallel_loader_fixtures(swift, path, threads):
+    """The parallel parser against the sequential (reference) reading on every
+    fixture, error cases included, in both value types."""
+    for dt in (swift.F64, swift.F32):
+        assert _same(swift.mmio_load(path, threads=1, dtype=dt),
+                     swift.mmio_load(path, threads=threads, dtype=dt))
+    assert _same(swift.mmio_load(path), swift.mmio_load(path, threads=0))
+
+
+def _write_mtx(path, field, sym, m, n, entries, rng, split=False, tail=""):
+    """Writes entries [(i, j, value token(s))] with uneven whitespace: tabs,
+    CRLF, blank lines, and (split) entries broken over lines."""
+    out = ["%%%%MatrixMarket matrix coordinate %s %s\n" % (field, sym), "% a comment\n",
+           "%d %d %d\n" % (m, n, len(entries))]
+    for k, (i, j, v) in enumerate(entries):
+        toks = [str(i), str(j)] + v
+        seps = [rng.choice([" ", "  ", "\t"]) for _ in toks[1:]]
+        if split and k % 7 == 3:
+            seps[-1] = "\n"
+        line = toks[0] + "".join(s + t for s, t in zip(seps, toks[1:]))
+        out.append(line + rng.choice(["\n", "\r\n", " \n", "\n\n"]))
+    out.append(tail)
+    with open(path, "w", newline="") as f:
+        f.write("".join(out))
+
+
+@pytest.mark.parametrize("kind", ["real", "real_split", "pattern_sym", "integer", "complex_herm",
+                                  "special_values", "trailing", "bad_token", "truncated",
+                                  "out_of_range"])
+def test_parallel_loader_generated(swift, tmp_path, kind):
+    rng = np.random.default_rng(hash(kind) % 2**32)
+    m, n, E = 3000, 2500, 60000
+    I = rng.integers(1, m + 1, E)
+    J = rng.integers(1, n + 1, E)
+    field, sym, split, tail = "real", "general", False, ""
+    vals = [["%.17g" % x] for x in rng.standard_normal(E)]
+    if kind == "real_split":
+        split = True
+    elif kind == "pattern_sym":
+        field, sym, n = "pattern", "symmetric", m
+        J = rng.integers(1, m + 1, E)
+        vals = [[] for _ in range(E)]
+    elif kind == "integer":
+        field = "integer"
+        vals = [[str(int(x))] for x in rng.integers(-1000, 1000, E)]
+    elif kind == "complex_herm":
+        field, sym, n = "complex", "hermitian", m
+        J = rng.integers(1, m + 1, E)
+        vals = [["%.6e" % a, "%.6e" % b] for a, b in rng.standard_normal((E, 2))]
+    elif kind == "special_values":
+        pool = ["1e-310", "-0.0", "inf", "-inf", "nan", "0x1.8p+1", "+3.25", "1E5", "7"]
+        vals = [[pool[k % len(pool)]] for k in range(E)]
+    elif kind == "trailing":
+        tail = "1 1 5.0\n2 2 zzz\n"
+    elif kind == "bad_token":
+        vals[E // 2] = ["1.5abc"]
+    elif kind == "out_of_range":
+        I[E - 5] = m + 1
+    entries = list(zip(I.tolist(), J.tolist(), vals))
+    p = str(tmp_path / (kind + ".mtx"))
+    _write_mtx(p, field, sym, m, n, entries, rng, split=split, tail=tail)
+    if kind == "truncated":
+        txt = open(p).read()
+        open(p, "w").write(txt[: int(len(txt) * 0.8)])
+    seq = swift.mmio_load(p, threads=1)
+    if kind in ("bad_token", "truncated", "out_of_range"):
+        assert seq[0] == -3
+    else:
+        assert seq[0] == 0 and seq[1]["nnz"] >= E
+    for t in (2, 5, 16):
+        assert _same(seq, swift.mmio_load(p, threads=t)), t
+    assert _same(swift.mmio_load(p, threads=1, dtype=swift.F32),
+                 swift.mmio_load(p, threads=16, dtype=swift.F32))
