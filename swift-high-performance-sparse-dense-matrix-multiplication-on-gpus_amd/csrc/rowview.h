@@ -411,16 +411,14 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 #ifndef SWIFT_UNROLL_WIDE
 #define SWIFT_UNROLL_WIDE 4
 #endif
-// A lane group of 64 lanes holds one entry per slot (1 KiB or wider B rows):
-// twice the slots keep as many entries in flight as the narrower shapes (not
-// in the window walk, whose 64-register budget would spill them)
-// (measured on the 1M-row matrix: FP64 K=128 1.72 -> 1.62 ms, K=256 3.81 ->
-// 3.40 ms; FP32 K=256 1.86 -> 1.78 ms, K=512 4.01 -> 3.47 ms).
-#ifndef SWIFT_UNROLL_EPS1
-#define SWIFT_UNROLL_EPS1 8
-#endif
-#ifndef SWIFT_EPS1_QMAX
-#define SWIFT_EPS1_QMAX 2
+// Lane groups of 32 or 64 lanes (one or two entries per slot, at most two
+// chunks per lane) take twice the slots, so a wave keeps 8-16 entries in
+// flight like the narrower shapes (not in the window walk, whose 64-register
+// budget would spill them).  Measured on the 1M-row matrix: FP32 K=128 0.879
+// -> 0.851 ms, K=512 4.01 -> 3.45 ms; FP64 K=128 1.72 -> 1.62 ms, K=256 3.81
+// -> 3.39 ms; BF16 K=128 0.516 -> 0.496 ms; K=32 (8-lane groups) was slower.
+#ifndef SWIFT_UNROLL_LONG
+#define SWIFT_UNROLL_LONG 8
 #endif
 
 //
@@ -433,9 +431,9 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
     static_assert(!WIN || (BUF && !MASK), "the window walk is a buffer-load, unmasked walk");
     constexpr int EPS = 64 / LPE;
     // B-row loads in flight per lane group and step
-    constexpr int UNROLL = sizeof(TB) == 2             ? SWIFT_UNROLL_NARROW
-                           : (LPE == 64 && Q <= SWIFT_EPS1_QMAX && !WIN) ? SWIFT_UNROLL_EPS1
-                                                   : SWIFT_UNROLL_WIDE;
+    constexpr int UNROLL = (LPE >= 32 && Q <= 2 && !WIN) ? SWIFT_UNROLL_LONG
+                           : sizeof(TB) == 2             ? SWIFT_UNROLL_NARROW
+                                                         : SWIFT_UNROLL_WIDE;
     constexpr int STEP = EPS * UNROLL;  // entries per iteration
     const int lane = threadIdx.x & 63;
     const int h = lane / LPE, sub = lane % LPE;
