@@ -313,16 +313,56 @@ def extras(S, torch, M, d, B, K):
     bf = time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16))
     res["bf16_row_ms"] = round(bf, 4)
     res["bf16_speedup_vs_f32"] = round(base / bf, 3)
+    res["bf16_error"] = lowp_error(torch, d, B, C)
     for name in ("ROWVIEW", "WINDOW"):
         k = getattr(S, "KERNEL_" + name)
         res["bf16_row_%s_ms" % name.lower()] = round(
             time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16, kernel=k)), 4)
+    M.prepare_values(S.F16)
+    Bf, sc = S.convert_dense(B, S.F16, col_scale=True)
+    f16 = time_call(torch, lambda: M.spmm_scaled(Bf, sc, out=C))
+    res["f16_scaled_row_ms"] = round(f16, 4)
+    res["f16_scaled_error"] = lowp_error(torch, d, B, C)
+    del Bf, sc
     dC = torch.randn((m, K), device="cuda")
     bw = time_call(torch, lambda: M.backward_fused(dC, B), reps=5)
     res["fwd_bwd_over_fwd"] = round((base + bw) / base, 3)
     res["bwd_fused_ms"] = round(bw, 4)
     res["f32_row_gflops"] = round(flops / base / 1e6, 1)
     return res
+
+
+def lowp_error(torch, d, B, C):
+    """Error of a 16-bit-storage product C (FP32) against an FP64 product of the
+    same matrix and B computed by torch's sparse CSR matmul on the GPU: the
+    largest per-row max relative error (against the row's max |C64|) and the
+    largest error normalised by (|A||B|)[i,k]."""
+    try:
+        return _lowp_error(torch, d, B, C)
+    except (RuntimeError, ValueError) as e:  # the FP64 reference could not run
+        return {"error": str(e)[:200]}
+
+
+def _lowp_error(torch, d, B, C):
+    dev = B.device
+    rp = torch.tensor(d["rowptr"].astype(np.int64), device=dev)
+    ci = torch.tensor(d["colidx"].astype(np.int64), device=dev)
+    # the FP32 values the matrix was built from
+    v = torch.tensor(d["val"].astype(np.float32).astype(np.float64), device=dev)
+    m, n = d["m"], d["n"]
+    A = torch.sparse_csr_tensor(rp, ci, v, (m, n))
+    Aabs = torch.sparse_csr_tensor(rp, ci, v.abs(), (m, n))
+    B64 = B.double()
+    C64 = A @ B64
+    bound = Aabs @ B64.abs()
+    err = (C.double() - C64).abs()
+    row = err.amax(1) / C64.abs().amax(1).clamp_min(1e-30)
+    out = {"row_max_rel_max": float(row.max()),
+           "row_max_rel_p99": float(torch.quantile(row.float(), 0.99)),
+           "normalised_max": float((err / (bound + 1e-30)).max()),
+           "reference": "torch sparse CSR matmul in FP64 on the GPU"}
+    del A, Aabs, C64, bound, err
+    return out
 
 
 def batched(S, torch, count, scale=1.0, reps=10):

@@ -259,10 +259,16 @@ __device__ __forceinline__ T multi_group_sum(const T (&d)[N], int sub) {
     return r;
 }
 
+// Slots per step of the backward walk.  8 for 32/64-lane groups (as in the
+// forward walk) was measured slower on the 1M-row matrix (fused 1.06 -> 1.30
+// ms at K=128), so every shape takes 4.
+#ifndef SWIFT_BW_UNROLL_LONG
+#define SWIFT_BW_UNROLL_LONG 4
+#endif
 template <class T, int VEC, int Q, int LPE, bool DB, bool DV, bool BUF>
 __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     constexpr int EPS = 64 / LPE;
-    constexpr int UNROLL = 4;
+    constexpr int UNROLL = (LPE >= 32 && Q <= 2) ? SWIFT_BW_UNROLL_LONG : 4;
     constexpr int STEP = EPS * UNROLL;
     const int lane = threadIdx.x & 63;
     const int h = lane / LPE, sub = lane % LPE;

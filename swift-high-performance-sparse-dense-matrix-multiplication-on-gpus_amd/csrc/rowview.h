@@ -299,18 +299,33 @@ __device__ __forceinline__ void store_vec(TC *p, const TC (&x)[VEC]) {
     }
 }
 
+// Output scales of a lane's elements (F16 storage: per-k scale times the
+// value scale), loaded once per walk: a lane's k positions never change.
+template <class TC, int VEC, int Q, int LPE, bool MASK>
+__device__ __forceinline__ void lane_scales(const RowArgs &a, int sub, TC (&osc)[Q][VEC]) {
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) {
+            const int k = q * LPE * VEC + sub * VEC + v;
+            osc[q][v] = (!MASK || k < a.kcount) ? out_scale<TC>(a, k) : (TC)0;
+        }
+}
+
 // Writes one finished row: plain (or read-modify-write when accumulating)
-// vector stores, or atomics for a row split across segments.
+// vector stores, or atomics for a row split across segments.  osc: the
+// lane's output scales when hoisted (lane_scales), else read per store.
 template <class TC, int VEC, int Q, int LPE, bool MASK>
 __device__ __forceinline__ void put_row(const RowArgs &a, int row, int sub, bool split,
-                                        TC (&acc)[Q][VEC]) {
+                                        TC (&acc)[Q][VEC], const TC (*osc)[VEC] = nullptr) {
     TC *crow = static_cast<TC *>(a.C) + (size_t)row * a.ldc + a.kbase;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int k0 = q * LPE * VEC + sub * VEC;
         TC x[VEC];
 #pragma unroll
-        for (int v = 0; v < VEC; v++) x[v] = acc[q][v] * out_scale<TC>(a, k0 + v);
+        for (int v = 0; v < VEC; v++)
+            x[v] = acc[q][v] * (osc ? osc[q][v] : out_scale<TC>(a, k0 + v));
         if (MASK) {
 #pragma unroll
             for (int v = 0; v < VEC; v++)
@@ -461,6 +476,10 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
     for (int q = 0; q < Q; q++)
 #pragma unroll
         for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+    // F16 storage scales its output per k: hoisted out of the row stores
+    constexpr bool HOIST = std::is_same<TB, __half>::value;
+    TC osc[Q][VEC];
+    if constexpr (HOIST) lane_scales<TC, VEC, Q, LPE, MASK>(a, sub, osc);
     while (true) {
         if (row - rbase >= 64) {
             rbase = row;
@@ -591,7 +610,8 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
 #pragma unroll
                 for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
         if (h == 0)
-            put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, last || (row == st.x && carried_in), acc);
+            put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, last || (row == st.x && carried_in), acc,
+                                           HOIST ? osc : nullptr);
         else
 #pragma unroll
             for (int q = 0; q < Q; q++)
