@@ -68,6 +68,20 @@ def keep_rows(d, parity):
     return out
 
 
+def host_cpu():
+    """CPU model and logical CPU count of the host the baseline ran on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(d, K, seed):
     """Reference Swift_cpu (src/Swift_cpu.h:31-65), 1 thread, on the same matrix."""
     import oracle as O
@@ -256,7 +270,7 @@ def main():
     if not args.no_cpu:
         t0 = time.time()
         out["cpu_baseline"] = cpu_baseline(d, K, seed)
-        out["cpu_baseline"]["cpu"] = platform.processor() or platform.machine()
+        out["cpu_baseline"].update(host_cpu())
         log("cpu baseline %.1fs: %s" % (time.time() - t0, out["cpu_baseline"]))
     print(json.dumps(out), flush=True)
 
