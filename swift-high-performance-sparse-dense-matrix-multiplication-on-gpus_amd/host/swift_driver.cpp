@@ -112,10 +112,77 @@ static int verify(const T *got, const T *golden, size_t n) {
 
 // rocSPARSE CSR SpMM with column-major B/C (the reference's cuSPARSE call,
 // src/gpu_cusparse_spmm.h:75-95); returns the median kernel time in ms.
-// row_major: B (n x K) and C (m x K) row-major instead (rocsparse_order_row).
+// One rocsparse_spmm configuration on device operands: descriptors, buffer,
+// preprocess, then `reps` timed compute stages after 2 warm-ups; the median
+// kernel time in ms, or -1 when rocSPARSE rejects the configuration (some
+// algorithms take only one dense order).
+template <class T>
+static float vendor_run(rocsparse_handle h, int m, int n, int nnz, int K, int *drp, int *dci,
+                        T *dv, T *dB, T *dC, bool row_major, rocsparse_spmm_alg alg, int reps) {
+    rocsparse_spmat_descr A = nullptr;
+    rocsparse_dnmat_descr Bd = nullptr, Cd = nullptr;
+    void *buf = nullptr;
+    float result = -1.0f;
+    do {
+        if (rocsparse_create_csr_descr(&A, m, n, nnz, drp, dci, dv, rocsparse_indextype_i32,
+                                       rocsparse_indextype_i32, rocsparse_index_base_zero,
+                                       Traits<T>::rt) != rocsparse_status_success)
+            break;
+        const rocsparse_order order = row_major ? rocsparse_order_row : rocsparse_order_column;
+        if (rocsparse_create_dnmat_descr(&Bd, n, K, row_major ? K : n, dB, Traits<T>::rt, order) !=
+                rocsparse_status_success ||
+            rocsparse_create_dnmat_descr(&Cd, m, K, row_major ? K : m, dC, Traits<T>::rt, order) !=
+                rocsparse_status_success)
+            break;
+        T alpha = 1, beta = 0;
+        size_t bytes = 0;
+        if (rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, Bd,
+                           &beta, Cd, Traits<T>::rt, alg, rocsparse_spmm_stage_buffer_size,
+                           &bytes, nullptr) != rocsparse_status_success)
+            break;
+        HIP_OK(hipMalloc(&buf, std::max<size_t>(bytes, 16)));
+        if (rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, Bd,
+                           &beta, Cd, Traits<T>::rt, alg, rocsparse_spmm_stage_preprocess, &bytes,
+                           buf) != rocsparse_status_success)
+            break;
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        std::vector<float> t;
+        bool ok = true;
+        for (int r = 0; r < reps + 2 && ok; r++) {
+            HIP_OK(hipEventRecord(e0, nullptr));
+            ok = rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A,
+                                Bd, &beta, Cd, Traits<T>::rt, alg, rocsparse_spmm_stage_compute,
+                                &bytes, buf) == rocsparse_status_success;
+            HIP_OK(hipEventRecord(e1, nullptr));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            if (r >= 2) t.push_back(ms);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (!ok || t.empty()) break;
+        std::sort(t.begin(), t.end());
+        result = t[t.size() / 2];
+    } while (false);
+    if (A) rocsparse_destroy_spmat_descr(A);
+    if (Bd) rocsparse_destroy_dnmat_descr(Bd);
+    if (Cd) rocsparse_destroy_dnmat_descr(Cd);
+    if (buf) (void)hipFree(buf);
+    return result;
+}
+
+// rocSPARSE SpMM (the reference compares against cuSPARSE SpMM, CSR with
+// column-major dense operands, src/gpu_cusparse_spmm.h:75-95) with algorithm
+// `alg`; B and C are column-major, or row-major (rocsparse_order_row) when
+// row_major.  Returns the median kernel time in ms (-1: not supported) and
+// leaves the product in Ccol (host, in the same order as B).
 template <class T>
 static float vendor_spmm(int m, int n, int nnz, int K, const int *rp, const int *ci,
-                         const T *val, const T *Bcol, T *Ccol, int reps, bool row_major = false) {
+                         const T *val, const T *Bcol, T *Ccol, int reps, bool row_major = false,
+                         rocsparse_spmm_alg alg = rocsparse_spmm_alg_default) {
     int *drp, *dci;
     T *dv, *dB, *dC;
     HIP_OK(hipMalloc(&drp, sizeof(int) * (m + 1)));
@@ -128,55 +195,15 @@ static float vendor_spmm(int m, int n, int nnz, int K, const int *rp, const int 
     HIP_OK(hipMemcpy(dv, val, sizeof(T) * nnz, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(dB, Bcol, sizeof(T) * (size_t)n * K, hipMemcpyHostToDevice));
     rocsparse_handle h;
-    RS_OK(rocsparse_create_handle(&h));
-    rocsparse_spmat_descr A;
-    rocsparse_dnmat_descr Bd, Cd;
-    RS_OK(rocsparse_create_csr_descr(&A, m, n, nnz, drp, dci, dv, rocsparse_indextype_i32,
-                                     rocsparse_indextype_i32, rocsparse_index_base_zero,
-                                     Traits<T>::rt));
-    if (row_major) {
-        RS_OK(rocsparse_create_dnmat_descr(&Bd, n, K, K, dB, Traits<T>::rt, rocsparse_order_row));
-        RS_OK(rocsparse_create_dnmat_descr(&Cd, m, K, K, dC, Traits<T>::rt, rocsparse_order_row));
-    } else {
-        RS_OK(rocsparse_create_dnmat_descr(&Bd, n, K, n, dB, Traits<T>::rt, rocsparse_order_column));
-        RS_OK(rocsparse_create_dnmat_descr(&Cd, m, K, m, dC, Traits<T>::rt, rocsparse_order_column));
+    float t = -1.0f;
+    if (rocsparse_create_handle(&h) == rocsparse_status_success) {
+        t = vendor_run<T>(h, m, n, nnz, K, drp, dci, dv, dB, dC, row_major, alg, reps);
+        rocsparse_destroy_handle(h);
     }
-    T alpha = 1, beta = 0;
-    size_t bytes = 0;
-    RS_OK(rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, Bd,
-                         &beta, Cd, Traits<T>::rt, rocsparse_spmm_alg_default,
-                         rocsparse_spmm_stage_buffer_size, &bytes, nullptr));
-    void *buf = nullptr;
-    HIP_OK(hipMalloc(&buf, std::max<size_t>(bytes, 16)));
-    RS_OK(rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, Bd,
-                         &beta, Cd, Traits<T>::rt, rocsparse_spmm_alg_default,
-                         rocsparse_spmm_stage_preprocess, &bytes, buf));
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
-    std::vector<float> t;
-    for (int r = 0; r < reps + 2; r++) {
-        HIP_OK(hipEventRecord(e0, nullptr));
-        RS_OK(rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A,
-                             Bd, &beta, Cd, Traits<T>::rt, rocsparse_spmm_alg_default,
-                             rocsparse_spmm_stage_compute, &bytes, buf));
-        HIP_OK(hipEventRecord(e1, nullptr));
-        HIP_OK(hipEventSynchronize(e1));
-        float ms;
-        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-        if (r >= 2) t.push_back(ms);
-    }
-    HIP_OK(hipMemcpy(Ccol, dC, sizeof(T) * (size_t)m * K, hipMemcpyDeviceToHost));
-    rocsparse_destroy_spmat_descr(A);
-    rocsparse_destroy_dnmat_descr(Bd);
-    rocsparse_destroy_dnmat_descr(Cd);
-    rocsparse_destroy_handle(h);
-    for (void *p : {buf, (void *)drp, (void *)dci, (void *)dv, (void *)dB, (void *)dC})
+    if (t >= 0) HIP_OK(hipMemcpy(Ccol, dC, sizeof(T) * (size_t)m * K, hipMemcpyDeviceToHost));
+    for (void *p : {(void *)drp, (void *)dci, (void *)dv, (void *)dB, (void *)dC})
         (void)hipFree(p);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    std::sort(t.begin(), t.end());
-    return t[t.size() / 2];
+    return t;
 }
 
 struct Args {
@@ -302,6 +329,10 @@ static float vendor_time(const Problem<T> &P, const Operands<T> &o, int reps) {
     std::vector<T> Ccol((size_t)P.m * o.K);
     float t = vendor_spmm<T>(P.m, P.n, P.nnz, o.K, P.rp, P.ci, P.val.data(), o.Bcol.data(),
                              Ccol.data(), reps);
+    if (t < 0) {
+        printf("error rocSPARSE SpMM: not supported\n");
+        return -1111;
+    }
     int bad = verify(Ccol.data(), o.Gcol.data(), Ccol.size());
     if (bad) {
         printf("error rocSPARSE SpMM, error = %d\n", bad);
@@ -311,20 +342,19 @@ static float vendor_time(const Problem<T> &P, const Operands<T> &o, int reps) {
     return t;
 }
 
-// rocSPARSE with row-major B and C (the layout of the row-major Swift
-// default) with ResultVerify; -1111 when it disagrees.
+// rocSPARSE with algorithm alg in either dense order, verified against the
+// host golden (ResultVerify's rule); -1 when rocSPARSE rejects the
+// configuration or the result disagrees.
 template <class T>
-static float vendor_time_row(const Problem<T> &P, const Operands<T> &o, int reps) {
+static float vendor_time_alg(const Problem<T> &P, const Operands<T> &o, int reps, bool row_major,
+                             rocsparse_spmm_alg alg) {
     std::vector<T> C((size_t)P.m * o.K);
-    float t = vendor_spmm<T>(P.m, P.n, P.nnz, o.K, P.rp, P.ci, P.val.data(), o.B.data(), C.data(),
-                             reps, true);
-    int bad = verify(C.data(), o.golden.data(), C.size());
-    if (bad) {
-        printf("error rocSPARSE SpMM (row-major), error = %d\n", bad);
-        return -1111;
-    }
-    printf("success rocSPARSE SpMM (row-major), process:%f ms\n", t);
-    return t;
+    const float t = vendor_spmm<T>(P.m, P.n, P.nnz, o.K, P.rp, P.ci, P.val.data(),
+                                   row_major ? o.B.data() : o.Bcol.data(), C.data(), reps,
+                                   row_major, alg);
+    if (t < 0) return -1;
+    const int bad = verify(C.data(), row_major ? o.golden.data() : o.Gcol.data(), C.size());
+    return bad ? -1 : t;
 }
 
 // One Swift variant on host pointers (swift_spmm_host: kernel-only events,
@@ -538,19 +568,40 @@ static int mode_preprocess(const Args &a, Problem<T> &P) {
     return 0;
 }
 
-// vendor: rocSPARSE SpMM alone, column-major (the reference's cuSPARSE
-// call, src/gpu_cusparse_spmm.h:75-95) and row-major, each verified against
-// the host golden; one line on stdout:
-// "vendor rocSPARSE m M n N width K nnz NNZ row-major T col-major T".
+// vendor: rocSPARSE SpMM alone, every CSR algorithm in both dense orders,
+// each verified against the host golden; one line per algorithm, then the
+// best per order:
+// "vendor rocSPARSE m M n N width K nnz NNZ row-major T col-major T
+//  row-alg NAME col-alg NAME" (T = -1: no algorithm passed).
 template <class T>
 static int mode_vendor(const Args &a, Problem<T> &P) {
     Operands<T> o;
     make_operands(P, a.K, o);
-    const float tc = vendor_time(P, o, a.reps);
-    const float tr = vendor_time_row(P, o, a.reps);
-    printf("vendor rocSPARSE m %d n %d width %d nnz %d row-major %f col-major %f\n", P.m, P.n,
-           a.K, P.nnz, tr, tc);
-    return (tc < 0 || tr < 0) ? 2 : 0;
+    const struct {
+        const char *name;
+        rocsparse_spmm_alg alg;
+    } algs[] = {{"default", rocsparse_spmm_alg_default},
+                {"csr", rocsparse_spmm_alg_csr},
+                {"csr_row_split", rocsparse_spmm_alg_csr_row_split},
+                {"csr_nnz_split", rocsparse_spmm_alg_csr_nnz_split},
+                {"csr_merge_path", rocsparse_spmm_alg_csr_merge_path}};
+    float best[2] = {-1, -1};
+    const char *best_name[2] = {"none", "none"};
+    for (const auto &g : algs) {
+        float t[2];
+        for (int r = 0; r < 2; r++) {
+            t[r] = vendor_time_alg(P, o, a.reps, r == 0, g.alg);
+            if (t[r] >= 0 && (best[r] < 0 || t[r] < best[r])) {
+                best[r] = t[r];
+                best_name[r] = g.name;
+            }
+        }
+        printf("vendor-alg %s row-major %f col-major %f\n", g.name, t[0], t[1]);
+    }
+    printf("vendor rocSPARSE m %d n %d width %d nnz %d row-major %f col-major %f row-alg %s "
+           "col-alg %s\n",
+           P.m, P.n, a.K, P.nnz, best[0], best[1], best_name[0], best_name[1]);
+    return (best[0] < 0 || best[1] < 0) ? 2 : 0;
 }
 
 // mtxinfo: sparsity and the share of empty 32x32 blocks with the product

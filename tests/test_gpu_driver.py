@@ -91,10 +91,15 @@ def test_mtxinfo_mode(tmp_path):
 
 @pytest.mark.parametrize("prec", ["--fp32", "--fp64"])
 def test_vendor_mode(tmp_path, prec):
-    """rocSPARSE alone, row- and column-major, both verified against the host
-    golden (the vendor leg of bench.py)."""
+    """rocSPARSE alone: every CSR algorithm in both dense orders, each verified
+    against the host golden, and the best per order (the vendor leg of
+    bench.py)."""
     out = run(["--gen", "1", "--scale", "0.25", prec, "--mode", "vendor", "32"], tmp_path)
     (v,) = [l.split() for l in out.splitlines() if l.startswith("vendor rocSPARSE")]
     assert v[10] == "row-major" and v[12] == "col-major"
+    assert v[14] == "row-alg" and v[16] == "col-alg"
     assert float(v[11]) > 0 and float(v[13]) > 0
-    assert "success rocSPARSE SpMM (row-major)" in out and "success rocSPARSE SpMM," in out
+    algs = [l.split() for l in out.splitlines() if l.startswith("vendor-alg")]
+    assert [a[1] for a in algs] == ["default", "csr", "csr_row_split", "csr_nnz_split",
+                                    "csr_merge_path"]
+    assert float(v[11]) == min(float(a[3]) for a in algs if float(a[3]) > 0)
