@@ -33,6 +33,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -142,6 +144,22 @@ inline bool token_double(const char *p, const char *q, double &v) {
     return *end == 0;
 }
 
+// Runs fn(0) .. fn(n-1) on n threads.  If a thread cannot be started, the
+// ones already running are joined before the error propagates (a joinable
+// std::thread must not be destroyed).
+template <class F>
+void run_threads(int n, F fn) {
+    std::vector<std::thread> th;
+    th.reserve((size_t)n);
+    try {
+        for (int t = 0; t < n; t++) th.emplace_back(fn, t);
+    } catch (...) {
+        for (auto &x : th) x.join();
+        throw;
+    }
+    for (auto &x : th) x.join();
+}
+
 // Outcome of the parallel parse: OK, a data error, or "use the sequential
 // parser" (a token that strtol/strtod do not consume whole).
 enum { PAR_OK = 0, PAR_DATA = 1, PAR_FALLBACK = 2 };
@@ -161,29 +179,22 @@ int parse_parallel(const char *d, const char *end, char field, bool mirror, int 
     }
     // pass 1: tokens per chunk
     std::vector<long long> ntok((size_t)nthreads + 1, 0);
-    {
-        std::vector<std::thread> th;
-        for (int t = 0; t < nthreads; t++)
-            th.emplace_back([&, t]() {
-                long long c = 0;
-                bool in = false;
-                for (const char *p = cut[t]; p < cut[t + 1]; p++) {
-                    const bool w = is_ws(*p);
-                    if (!w && !in) c++;
-                    in = !w;
-                }
-                ntok[t + 1] = c;
-            });
-        for (auto &x : th) x.join();
-    }
+    run_threads(nthreads, [&](int t) {
+        long long c = 0;
+        bool in = false;
+        for (const char *p = cut[t]; p < cut[t + 1]; p++) {
+            const bool w = is_ws(*p);
+            if (!w && !in) c++;
+            in = !w;
+        }
+        ntok[t + 1] = c;
+    });
     for (int t = 0; t < nthreads; t++) ntok[t + 1] += ntok[t];
     if (ntok[nthreads] < NZ * tpe) return PAR_DATA;  // fscanf would meet the end
     // pass 2: entries whose first token is in the chunk
     std::atomic<int> status{PAR_OK};
     {
-        std::vector<std::thread> th;
-        for (int t = 0; t < nthreads; t++)
-            th.emplace_back([&, t]() {
+        run_threads(nthreads, [&](int t) {
                 long long g = ntok[t];  // global index of the next token
                 const char *p = cut[t];
                 auto next = [&](const char *&a, const char *&b) -> bool {
@@ -221,7 +232,6 @@ int parse_parallel(const char *d, const char *end, char field, bool mirror, int 
                     E.vv[e] = x;
                 }
             });
-        for (auto &x : th) x.join();
     }
     return status.load();
 }
@@ -265,8 +275,12 @@ int load(int *m_out, int *n_out, int *nnz_out, int *sym_out, int **rowptr_out,
         void *map = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fileno(f), 0);
         if (map != MAP_FAILED) {
             const char *base = static_cast<const char *>(map);
-            par = parse_parallel(base + start, base + st.st_size, field, mirror, M, N, NZ,
-                                 threads, E);
+            try {
+                par = parse_parallel(base + start, base + st.st_size, field, mirror, M, N, NZ,
+                                     threads, E);
+            } catch (const std::exception &) {  // threads could not be started
+                par = PAR_FALLBACK;
+            }
             munmap(map, (size_t)st.st_size);
         }
     }
@@ -321,28 +335,41 @@ int load(int *m_out, int *n_out, int *nnz_out, int *sym_out, int **rowptr_out,
     return SWIFT_OK;
 }
 
+
+// The C entry points: host allocation failures become SWIFT_ERR_ALLOC
+// instead of escaping through C.
+template <class T>
+int load_c(int *m, int *n, int *nnz, int *sym, int **rp, int **ci, T **v, const char *path,
+           int nthreads) {
+    try {
+        return load<T>(m, n, nnz, sym, rp, ci, v, path, nthreads);
+    } catch (const std::bad_alloc &) {
+        return SWIFT_ERR_ALLOC;
+    }
+}
+
 }  // namespace
 
 extern "C" int swift_mmio_allinone(int *m, int *n, int *nnz, int *isSymmetric, int **csrrowptr,
                                    int **csrcolidx, double **csrval, const char *filename) {
-    return load<double>(m, n, nnz, isSymmetric, csrrowptr, csrcolidx, csrval, filename, 0);
+    return load_c<double>(m, n, nnz, isSymmetric, csrrowptr, csrcolidx, csrval, filename, 0);
 }
 
 extern "C" int swift_mmio_allinone_f32(int *m, int *n, int *nnz, int *isSymmetric,
                                        int **csrrowptr, int **csrcolidx, float **csrval,
                                        const char *filename) {
-    return load<float>(m, n, nnz, isSymmetric, csrrowptr, csrcolidx, csrval, filename, 0);
+    return load_c<float>(m, n, nnz, isSymmetric, csrrowptr, csrcolidx, csrval, filename, 0);
 }
 
 extern "C" int swift_mmio_allinone_ex(int *m, int *n, int *nnz, int *isSymmetric,
                                       int **csrrowptr, int **csrcolidx, void **csrval,
                                       swift_dtype vtype, const char *filename, int nthreads) {
     if (vtype == SWIFT_F64)
-        return load<double>(m, n, nnz, isSymmetric, csrrowptr, csrcolidx,
-                            reinterpret_cast<double **>(csrval), filename, nthreads);
+        return load_c<double>(m, n, nnz, isSymmetric, csrrowptr, csrcolidx,
+                              reinterpret_cast<double **>(csrval), filename, nthreads);
     if (vtype == SWIFT_F32)
-        return load<float>(m, n, nnz, isSymmetric, csrrowptr, csrcolidx,
-                           reinterpret_cast<float **>(csrval), filename, nthreads);
+        return load_c<float>(m, n, nnz, isSymmetric, csrrowptr, csrcolidx,
+                             reinterpret_cast<float **>(csrval), filename, nthreads);
     return SWIFT_ERR_ARG;
 }
 
