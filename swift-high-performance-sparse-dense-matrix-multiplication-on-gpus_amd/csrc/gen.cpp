@@ -5,6 +5,7 @@
 // reproducible; these generators give the configurations a fixed, documented
 // shape.  PRNG: splitmix64-seeded xoshiro256**, one stream per row block so the
 // output does not depend on thread count.
+#include <new>
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -299,17 +300,21 @@ extern "C" int swift_generate(int which, int index, double scale, uint64_t seed,
                               int *nnz, int **rowptr, int **colidx, double **val, int *k_hint) {
     if (!m || !n || !nnz || !rowptr || !colidx || !val || !(scale > 0.0) || scale > 1.0)
         return SWIFT_ERR_ARG;
-    Csr A;
-    int k = 128;
-    switch (which) {
-        case 1: gen1(scale, seed, A); k = 32; break;
-        case 2: gen2(scale, seed, A); break;
-        case 4: gen4(scale, seed, A); break;
-        case 5: gen5(index, scale, seed, A, &k); break;
-        default: return SWIFT_ERR_ARG;
+    try {  // host allocation failures must not escape through C
+        Csr A;
+        int k = 128;
+        switch (which) {
+            case 1: gen1(scale, seed, A); k = 32; break;
+            case 2: gen2(scale, seed, A); break;
+            case 4: gen4(scale, seed, A); break;
+            case 5: gen5(index, scale, seed, A, &k); break;
+            default: return SWIFT_ERR_ARG;
+        }
+        if (k_hint) *k_hint = k;
+        return emit(A, m, n, nnz, rowptr, colidx, val);
+    } catch (const std::bad_alloc &) {
+        return SWIFT_ERR_ALLOC;
     }
-    if (k_hint) *k_hint = k;
-    return emit(A, m, n, nnz, rowptr, colidx, val);
 }
 
 extern "C" int swift_generate_dense(size_t count, int dist, double lo, double hi, uint64_t seed,
