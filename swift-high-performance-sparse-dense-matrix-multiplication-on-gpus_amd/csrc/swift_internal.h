@@ -60,10 +60,16 @@ inline int seg_class(size_t row_bytes, size_t elem) {
     return 4;
 }
 
-// Device workspace that grows on demand; owned by a matrix or a call site.
+// Device workspace that grows on demand; owned by a matrix or a call site
+// and released when it goes out of scope (hipFree synchronises the device,
+// so work still queued on it finishes first).  Not copyable.
 struct Buffer {
     void *ptr = nullptr;
     size_t bytes = 0;
+    Buffer() = default;
+    Buffer(const Buffer &) = delete;
+    Buffer &operator=(const Buffer &) = delete;
+    ~Buffer() { release(); }
     int alloc(size_t n);   // (re)allocates only when n > bytes
     void release();
     template <class T> T *as() const { return static_cast<T *>(ptr); }

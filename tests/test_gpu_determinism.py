@@ -52,3 +52,35 @@ def test_build_and_product_repeat(swift, cuda, name):
     Aabs.close()
     A1.close()
     A2.close()
+
+
+def test_no_device_memory_growth(swift, cuda):
+    """Build, transpose, statistics, 16-bit preparation and conversion and a
+    product, four times over: the device memory the library takes for its
+    temporaries comes back (free memory is measured by the HIP runtime, so
+    allocations outside torch count)."""
+    import torch
+    d = swift.generate(2, scale=0.1, seed=0)  # 100k x 100k, ~1.6M nonzeros
+    vals = d["val"].astype(np.float32)
+
+    def cycle():
+        M = swift.SwiftMatrix(d["m"], d["n"], d["rowptr"], d["colidx"], vals)
+        M.transpose()
+        M.stats()
+        M.prepare_values(swift.F16)
+        B = torch.rand((d["n"], 64), device="cuda")
+        Bh, sc = swift.convert_dense(B, swift.F16, col_scale=True)
+        M.spmm_scaled(Bh, sc)
+        M.spmm(B)
+        torch.cuda.synchronize()
+        M.close()
+
+    cycle()
+    torch.cuda.empty_cache()
+    free0 = torch.cuda.mem_get_info()[0]
+    for _ in range(4):
+        cycle()
+    torch.cuda.empty_cache()
+    free1 = torch.cuda.mem_get_info()[0]
+    print("free before %d after %d (MiB)" % (free0 >> 20, free1 >> 20), flush=True)
+    assert free0 - free1 < (64 << 20)
