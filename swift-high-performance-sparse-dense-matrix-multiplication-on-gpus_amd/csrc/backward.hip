@@ -259,6 +259,86 @@ __device__ __forceinline__ T multi_group_sum(const T (&d)[N], int sub) {
     return r;
 }
 
+// N slot rows of one buffer-load step of the backward walk, without branches
+// (only the last slot row can reach past the row's end; its unused slots
+// read zeros through OOB_OFFSET), as in the forward walk's gather_body.
+// DB: acc += v * G[c] for the row's dB.  DV: dvals[src] = <G[c], H[row]>, the
+// slots' partial dots summed over the lane group together (lane sub then
+// holds slot sub / (LPE / UNROLL)).
+template <class T, int VEC, int Q, int LPE, bool DB, bool DV, int UNROLL, int N>
+__device__ __forceinline__ void bw_body(const BwArgs &a, __amdgpu_buffer_rsrc_t rsrc, int base,
+                                        int j, int nb, int h, int sub, int mc, int ms, T mv,
+                                        const T (&hr)[Q][VEC], T (&acc)[Q][VEC]) {
+    constexpr int EPS = 64 / LPE;
+    T g[N][Q][VEC];
+    T vv[N];
+    int src[N];
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+        // one permute base per step, G rows by 24-bit offsets
+        const int addr = base + ((u * EPS) << 2);
+        const int c = __builtin_amdgcn_ds_bpermute(addr, mc);
+        src[u] = __builtin_amdgcn_ds_bpermute(addr, ms);
+        const T vs = bperm_val(addr, mv);
+        const bool ok = u < N - 1 || j + u * EPS + h < nb;
+        vv[u] = ok ? vs : (T)0;
+        const unsigned roff = __umul24((unsigned)c, a.gstride);
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            load_b_buf<T, VEC, T>(
+                rsrc,
+                ok ? roff + (unsigned)((q * LPE * VEC + sub * VEC) * (int)sizeof(T)) : OOB_OFFSET,
+                g[u][q]);
+    }
+    if (DB) {
+#pragma unroll
+        for (int u = 0; u < N; u++)
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int v = 0; v < VEC; v++) acc[q][v] += vv[u] * g[u][q][v];
+    }
+    if (DV) {
+        constexpr int LEAD = LPE / UNROLL;
+        T d[UNROLL];
+        int sv = 0;
+        const int us = sub / LEAD;
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) d[u] = (T)0;
+#pragma unroll
+        for (int u = 0; u < N; u++) {
+            T x = (T)0;
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int v = 0; v < VEC; v++) x += g[u][q][v] * hr[q][v];
+            d[u] = x;
+            if (us == u) sv = src[u];
+        }
+        const T r = multi_group_sum<T, LPE, UNROLL>(d, sub);
+        // slots past the row's end (us >= N among them) write nothing
+        if (sub % LEAD == 0 && j + us * EPS + h < nb) static_cast<T *>(a.dvals)[sv] = r;
+    }
+}
+
+// The body for nu = min(UNROLL, slot rows left) slot rows (wave-uniform).
+template <class T, int VEC, int Q, int LPE, bool DB, bool DV, int UNROLL, int N>
+__device__ __forceinline__ void bw_pick(int nu, const BwArgs &a, __amdgpu_buffer_rsrc_t rsrc,
+                                        int base, int j, int nb, int h, int sub, int mc, int ms,
+                                        T mv, const T (&hr)[Q][VEC], T (&acc)[Q][VEC]) {
+    if constexpr (N == 1) {
+        bw_body<T, VEC, Q, LPE, DB, DV, UNROLL, 1>(a, rsrc, base, j, nb, h, sub, mc, ms, mv, hr,
+                                                   acc);
+    } else {
+        if (nu >= N)
+            bw_body<T, VEC, Q, LPE, DB, DV, UNROLL, N>(a, rsrc, base, j, nb, h, sub, mc, ms, mv,
+                                                       hr, acc);
+        else
+            bw_pick<T, VEC, Q, LPE, DB, DV, UNROLL, N - 1>(nu, a, rsrc, base, j, nb, h, sub, mc,
+                                                           ms, mv, hr, acc);
+    }
+}
+
 // Slots per step of the backward walk.  8 for 32/64-lane groups (as in the
 // forward walk) was measured slower on the 1M-row matrix (fused 1.06 -> 1.30
 // ms at K=128), so every shape takes 4.
@@ -328,31 +408,14 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 T vv[UNROLL];
                 int src[UNROLL];
                 if constexpr (BUF) {
-                    // as in the forward walk: one permute base per step, G rows
-                    // by 24-bit offsets, unused slots read zeros out of range
-                    const int base = (off + j + h) << 2;
-#pragma unroll
-                    for (int u = 0; u < UNROLL; u++) {
-                        // slots wholly past the row's end issue nothing
-                        if (u > 0 && j + u * EPS >= nb) break;
-                        const int addr = base + ((u * EPS) << 2);
-                        const int c = __builtin_amdgcn_ds_bpermute(addr, mc);
-                        src[u] = __builtin_amdgcn_ds_bpermute(addr, ms);
-                        const T vs = bperm_val(addr, mv);
-                        const bool ok = j + u * EPS + h < nb;
-                        vv[u] = ok ? vs : (T)0;
-                        const unsigned roff = __umul24((unsigned)c, a.gstride);
-#pragma unroll
-                        for (int q = 0; q < Q; q++)
-                            load_b_buf<T, VEC, T>(
-                                rsrc,
-                                ok ? roff + (unsigned)((q * LPE * VEC + sub * VEC) * (int)sizeof(T))
-                                   : OOB_OFFSET,
-                                g[u][q]);
-                    }
+                    const int nu = min(UNROLL, (nb - j + EPS - 1) / EPS);
+                    bw_pick<T, VEC, Q, LPE, DB, DV, UNROLL, UNROLL>(nu, a, rsrc, (off + j + h) << 2,
+                                                                   j, nb, h, sub, mc, ms, mv, hr,
+                                                                   acc);
+                    continue;
                 }
 #pragma unroll
-                for (int u = 0; u < UNROLL && !BUF; u++) {
+                for (int u = 0; u < UNROLL; u++) {
                     const int idx = j + u * EPS + h;
                     // unconditional shuffles: a lane reading a masked-off source
                     // lane would get a stale value
@@ -372,7 +435,6 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 if (DB) {
 #pragma unroll
                     for (int u = 0; u < UNROLL; u++) {
-                        if (BUF && u > 0 && j + u * EPS >= nb) break;  // not loaded
 #pragma unroll
                         for (int q = 0; q < Q; q++)
 #pragma unroll
@@ -389,13 +451,11 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
 #pragma unroll
                     for (int u = 0; u < UNROLL; u++) {
                         T x = (T)0;
-                        if (!(BUF && u > 0 && j + u * EPS >= nb)) {
 #pragma unroll
-                            for (int q = 0; q < Q; q++)
+                        for (int q = 0; q < Q; q++)
 #pragma unroll
-                                for (int v = 0; v < VEC; v++) x += g[u][q][v] * hr[q][v];
-                            if (us == u) sv = src[u];
-                        }
+                            for (int v = 0; v < VEC; v++) x += g[u][q][v] * hr[q][v];
+                        if (us == u) sv = src[u];
                         d[u] = x;
                     }
                     const T r = multi_group_sum<T, LPE, UNROLL>(d, sub);
