@@ -32,7 +32,7 @@ ROW_KERNELS = {"ROW_FLAT": "KERNEL_FLAT", "ROW_WIN": "KERNEL_WINDOW"}
 
 @pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "equalrows_100x70",
                                   "narrow_50x20", "banded_3000"])
-@pytest.mark.parametrize("K", [32, 128, 256])
+@pytest.mark.parametrize("K", [32, 100, 128, 256])
 @pytest.mark.parametrize("layout", ["ROW", "COL", "ROW_FLAT", "ROW_WIN"])
 def test_bf16_storage(swift, oracle, cuda, name, K, layout):
     import torch
@@ -49,6 +49,49 @@ def test_bf16_storage(swift, oracle, cuda, name, K, layout):
     print("bf16", name, K, layout, "err", err,
           "row_max_rel", float(np.max(gpu_common.row_max_rel(C, C64))), flush=True)
     assert err <= 1e-2
+    M.close()
+
+
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "banded_3000", "equalrows_100x70"])
+@pytest.mark.parametrize("K", [32, 100, 128])
+@pytest.mark.parametrize("form", ["f32", "bf16"])
+def test_nonfinite_propagation(swift, cuda, name, K, form):
+    """inf / -inf / NaN in B reach exactly the outputs they reach in the plain sum: the
+    BF16 walk multiplies two entries' values per instruction, and unused slots read
+    zeros, so neither may leak a non-finite value into another output or lose one."""
+    import torch
+    import scipy.sparse as sp
+    m, n, rp, ci, v = CASES[name]
+    rng = np.random.default_rng(7)
+    B = rng.uniform(-1, 1, (n, K)).astype(np.float32)
+    cols = np.asarray(ci)
+    for x in (np.inf, -np.inf, np.nan):
+        for _ in range(3):  # B rows that some entry gathers
+            B[cols[rng.integers(cols.size)], rng.integers(K)] = x
+    vals = np.asarray(v, np.float32)
+    M = swift.SwiftMatrix(m, n, rp, ci, vals, dtype=torch.float32)
+    Bt = torch.tensor(B, device="cuda")
+    if form == "bf16":
+        M.prepare_values(swift.BF16)
+        C = M.spmm(swift.convert_dense(Bt, swift.BF16), dtype=swift.BF16).cpu().numpy()
+        # the reference rounds both operands the same way (to nearest even)
+        vq = torch.tensor(vals).to(torch.bfloat16).double().numpy()
+        Bq = Bt.cpu().to(torch.bfloat16).double().numpy()
+    else:
+        C = M.spmm(Bt).cpu().numpy()
+        vq, Bq = vals.astype(np.float64), B.astype(np.float64)
+    A = sp.csr_matrix((vq, np.asarray(ci), np.asarray(rp)), shape=(m, n))
+    with np.errstate(invalid="ignore"):
+        R = A @ Bq
+        absR = abs(A) @ np.abs(np.nan_to_num(Bq, nan=0.0, posinf=0.0, neginf=0.0))
+    assert np.array_equal(np.isnan(C), np.isnan(R))
+    assert np.array_equal(np.isposinf(C), np.isposinf(R))
+    assert np.array_equal(np.isneginf(C), np.isneginf(R))
+    fin = np.isfinite(R)
+    assert not fin.all()
+    # both sides sum the same (rounded) operands: only the FP32 accumulation differs
+    err = float(np.max(np.abs(C[fin] - R[fin]) / (absR[fin] + 1e-30)))
+    assert err <= 1e-5, err
     M.close()
 
 
