@@ -96,3 +96,35 @@ def test_large_b_sddmm(swift, cuda):
     M.close()
     del B, dC
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_large_b_batched(swift, cuda, graph):
+    """A batch holding one 4.4M-row B (K=128, 2.25 GB) walks every matrix with
+    64-bit loads; a small matrix beside it must come out the same."""
+    import torch
+    shapes = [(3000, 4_400_000, 128), (500, 700, 32)]
+    mats, Bs, refs = [], [], []
+    g = torch.Generator(device="cuda").manual_seed(3)
+    for i, (m, n, K) in enumerate(shapes):
+        rp, ci, v = _matrix(m, n, 8, seed=11 + i)
+        mats.append(swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float32))
+        B = torch.rand((n, K), device="cuda", generator=g) * 2 - 1
+        Bs.append(B)
+        refs.append(_reference(torch, m, rp, ci, v, B))
+    Cs = [torch.full((m, K), float("nan"), device="cuda") for m, _, K in shapes]
+    batch = swift.Batch(mats, [K for _, _, K in shapes])
+    for _ in range(2 if graph else 1):  # a graph is captured once, then replayed
+        for c in Cs:
+            c.fill_(float("nan"))
+        batch.spmm(Bs, Cs, graph=graph)
+        torch.cuda.synchronize()
+        for (R, Rabs), C in zip(refs, Cs):
+            err = float(((C.double() - R).abs() / (Rabs + 1e-30)).max())
+            print("large batched graph=%s err %g" % (graph, err), flush=True)
+            assert err <= 1e-5
+    batch.close()
+    for M in mats:
+        M.close()
+    del Bs, Cs
+    torch.cuda.empty_cache()
