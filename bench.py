@@ -156,6 +156,9 @@ def main():
                     help="row-major kernel family (DEFAULT, ROWVIEW, WINDOW, FLAT, BALANCED)")
     ap.add_argument("--part", choices=("all", "banded", "zipf"), default="all",
                     help="diagnostics: keep only the banded (even) or the Zipf (odd) rows")
+    ap.add_argument("--storage", choices=("f32", "bf16"), default="f32",
+                    help="diagnostics: time the BF16 storage form (A's values and B in BF16, "
+                         "FP32 accumulation and C) as the step")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the Swift SpMM path runs on a single MI355X "
@@ -198,8 +201,16 @@ def main():
     stream = torch.cuda.current_stream()
 
     kern = getattr(S, "KERNEL_" + args.kernel.upper())
+    bf16 = args.storage == "bf16"
+    if bf16:
+        M.prepare_values(S.BF16)
+        Bs = S.convert_dense(B, S.BF16)
+        step = lambda: M.spmm(Bs, out=C, dtype=S.BF16, kernel=kern)
+    else:
+        Bs = B
+        step = lambda: M.spmm(B, out=C, kernel=kern)
     for _ in range(args.warmup):
-        M.spmm(B, out=C, kernel=kern)
+        step()
     torch.cuda.synchronize()
 
     # whole-job throughput: K steps bracketed by synchronisation
@@ -208,7 +219,7 @@ def main():
     t0 = time.perf_counter()
     e0.record(stream)
     for _ in range(args.steps):
-        M.spmm(B, out=C, kernel=kern)
+        step()
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -219,16 +230,18 @@ def main():
         (ms_step, wall * 1e3 / args.steps, value))
 
     # per-kernel launch times (HIP events between the launches, same stream)
-    phases = np.array([M.spmm_timed(B, C, kernel=kern)
+    phases = np.array([M.spmm_timed(Bs, C, kernel=kern)
                        for _ in range(max(5, min(args.steps, 20)))])
     zero_ms, reg_ms, res_ms = [float(x) for x in np.median(phases, axis=0)]
     log("phases (median ms): zero %.4f regular %.4f reside %.4f" % (zero_ms, reg_ms, res_ms))
-    alg = algorithmic_bytes(m, n, nnz, K)
+    s_lp = 2 if bf16 else 4
+    alg = algorithmic_bytes(m, n, nnz, K, s_a=s_lp, s_b=s_lp)
     achieved = alg / (reg_ms * 1e-3)
-    # the committed counter summary covers the default workload only
+    # the committed counter summaries cover the full-size K=128 workload only
     default = args.scale == 1.0 and K == 128 and args.part == "all"
     traffic = measured_traffic(os.path.join(
-        ROOT, "profiles", "rowview_f32_k128_1Mx1M_16Mnnz_traffic.txt")) if default else None
+        ROOT, "profiles", "rowview_%s_k128_1Mx1M_16Mnnz_traffic.txt" % args.storage)
+    ) if default else None
     out = {
         "metric": "SpMM GFLOP/s + % HBM roofline (K=128 fp32); bf16 & fwd+bwd vs plain fp32",
         "value": round(value, 2),
@@ -240,11 +253,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.storage,
         "data": "synthetic (seeded generator, seed 0)",
         "config": {"workload": "Swift SpMM C=A*B, A %dx%d with %d nnz (even rows banded bw 64 "
-                               "8..32 nnz, odd rows Zipf(1.8) <=20k), FP32, K=%d, row-major B "
-                               "U[-1,1] in original column order" % (m, n, nnz, K),
+                               "8..32 nnz, odd rows Zipf(1.8) <=20k), %s, K=%d, row-major B "
+                               "U[-1,1] in original column order" % (
+                                   m, n, nnz, "BF16 storage of A's values and B, FP32 "
+                                   "accumulation and C" if bf16 else "FP32", K),
                    "rows": m, "cols": n, "nnz": nnz, "K": K, "layout": "row-major",
                    "rows_kept": args.part, "kernel": args.kernel.upper(),
                    # wall time of the constructor: CSR upload from pageable host memory,
@@ -264,6 +279,10 @@ def main():
                      "algorithmic_bytes": alg},
         "gpu": torch.cuda.get_device_name(0),
     }
+    if args.part == "all" and not bf16:
+        t0 = time.time()
+        out["forms"] = headline_forms(S, torch, M, d, B, K, ms_step)
+        log("forms %.1fs: %s" % (time.time() - t0, out["forms"]))
     if args.extras:
         out["extras"] = extras(S, torch, M, d, B, K)
     M.close()
@@ -297,8 +316,48 @@ def time_call(torch, fn, reps=10, warm=2):
     return e0.elapsed_time(e1) / reps
 
 
+def headline_forms(S, torch, M, d, B, K, f32_ms):
+    """The metric's two ratios on the same matrix and B.  BF16 storage (A's
+    values and B in BF16, FP32 accumulation and C) against the plain FP32 step,
+    with its error against FP64 and its regular kernel against the HBM roofline;
+    and the fused backward (dB = A^T dC and dvals = SDDMM(dC, B)) against the
+    forward alone."""
+    m, n, nnz = d["m"], d["n"], d["nnz"]
+    C = torch.empty((m, K), device="cuda", dtype=torch.float32)
+    M.prepare_values(S.BF16)
+    Bh = S.convert_dense(B, S.BF16)
+    bf = time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16), reps=20)
+    reg = float(np.median([M.spmm_timed(Bh, C)[1] for _ in range(10)]))
+    err = lowp_error(torch, d, B, C)
+    alg = algorithmic_bytes(m, n, nnz, K, s_a=2, s_b=2)
+    traffic = measured_traffic(os.path.join(
+        ROOT, "profiles", "rowview_bf16_k128_1Mx1M_16Mnnz_traffic.txt")) if K == 128 else None
+    del Bh
+    torch.cuda.empty_cache()
+    dC = torch.randn((m, K), device="cuda")
+    bw = time_call(torch, lambda: M.backward_fused(dC, B), reps=10)
+    del dC, C
+    torch.cuda.empty_cache()
+    return {
+        "bf16": {"ms_per_step": round(bf, 4), "speedup_vs_f32": round(f32_ms / bf, 3),
+                 "gflops": round(2.0 * nnz * K / bf / 1e6, 1),
+                 "regular_ms": round(reg, 4),
+                 "roofline": {"achieved": round(alg / (reg * 1e-3) / 1e9, 1),
+                              "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                              "frac": round(alg / (reg * 1e-3) / HBM_PEAK, 4),
+                              "algorithmic_bytes": alg, "traffic": traffic,
+                              "traffic_frac": (round(traffic / (reg * 1e-3) / HBM_PEAK, 4)
+                                               if traffic else None)},
+                 "error": err},
+        "fwd_bwd": {"fwd_ms": round(f32_ms, 4), "bwd_fused_ms": round(bw, 4),
+                    "fwd_bwd_over_fwd": round((f32_ms + bw) / f32_ms, 3)},
+    }
+
+
 def extras(S, torch, M, d, B, K):
-    """Other forms on the same matrix: layouts, sorted B, BF16 storage, FP64."""
+    """Other forms on the same matrix: the other row kernels, B in sorted row
+    order, column-major B, host buffers over PCIe, the other kernels' BF16
+    walks, and F16 storage with per-column scales."""
     m, n, nnz = d["m"], d["n"], d["nnz"]
     flops = 2.0 * nnz * K
     res = {}
@@ -333,10 +392,7 @@ def extras(S, torch, M, d, B, K):
     del Bhost, Chost, Bdev
     M.prepare_values(S.BF16)
     Bh = S.convert_dense(B, S.BF16)
-    bf = time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16))
-    res["bf16_row_ms"] = round(bf, 4)
-    res["bf16_speedup_vs_f32"] = round(base / bf, 3)
-    res["bf16_error"] = lowp_error(torch, d, B, C)
+    # (the default BF16 form and the backward are timed in the line's "forms")
     for name in ("ROWVIEW", "WINDOW"):
         k = getattr(S, "KERNEL_" + name)
         res["bf16_row_%s_ms" % name.lower()] = round(
@@ -347,10 +403,6 @@ def extras(S, torch, M, d, B, K):
     res["f16_scaled_row_ms"] = round(f16, 4)
     res["f16_scaled_error"] = lowp_error(torch, d, B, C)
     del Bf, sc
-    dC = torch.randn((m, K), device="cuda")
-    bw = time_call(torch, lambda: M.backward_fused(dC, B), reps=5)
-    res["fwd_bwd_over_fwd"] = round((base + bw) / base, 3)
-    res["bwd_fused_ms"] = round(bw, 4)
     res["f32_row_gflops"] = round(flops / base / 1e6, 1)
     return res
 
