@@ -242,7 +242,11 @@ int swift_permute_dense(const swift_matrix *mat, int K, swift_layout layout,
  * (src/Swift_3.h:216) and Swift_GPU_4 (src/Swift_4.h:305), minus their
  * host copies: B (n x K) and C (m x K) are device resident in `layout`.
  * dtype F32/F64 must equal the build's value type; BF16/F16 need
- * swift_prepare_values first and take B in that type, C in F32. */
+ * swift_prepare_values first and take B in that type, C in F32.
+ * Column-major products with SWIFT_KERNEL_DEFAULT transpose B and C through a
+ * workspace held by the handle: such calls on one handle must not overlap in
+ * time (issue them on one stream, or synchronise between streams).  Row-major
+ * calls share nothing and may overlap. */
 int swift_spmm(const swift_matrix *mat, int K, swift_layout layout,
                swift_dtype dtype, swift_kernel kernel, unsigned flags,
                const void *d_B, void *d_C, swift_stream_t stream);
