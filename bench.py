@@ -451,6 +451,7 @@ def batched(S, torch, count, scale=1.0, reps=10):
     mats, Ks, Bs, Cs = [], [], [], []
     flops = 0.0
     nnz_total = 0
+    alg = 0
     g = torch.Generator(device="cuda").manual_seed(0)
     for i in range(count):
         d = S.generate(5, scale=scale, seed=0, index=i)
@@ -463,6 +464,7 @@ def batched(S, torch, count, scale=1.0, reps=10):
         Cs.append(torch.empty((d["m"], K), device="cuda"))
         flops += 2.0 * d["nnz"] * K
         nnz_total += d["nnz"]
+        alg += algorithmic_bytes(d["m"], d["n"], d["nnz"], K)
     torch.cuda.synchronize()
     log("batched: %d matrices, %d nnz, built in %.1fs" % (count, nnz_total, time.time() - t0))
     stream = S._stream()
@@ -475,6 +477,7 @@ def batched(S, torch, count, scale=1.0, reps=10):
             L.swift_spmm(h, K, S.ROW_MAJOR, S.F32, S.KERNEL_DEFAULT, 0, b, c, stream)
 
     res = {"matrices": count, "nnz": nnz_total, "gflop": round(flops / 1e9, 3)}
+    log("batched: algorithmic bytes %d" % alg)
     res["separate_default_ms"] = round(time_call(torch, separate, reps=reps), 4)
     t0 = time.time()
     batch = S.Batch(mats, Ks)
@@ -487,6 +490,9 @@ def batched(S, torch, count, scale=1.0, reps=10):
     for k in ("separate_default", "separate_tuned", "batch", "graph"):
         res[k + "_gflops"] = round(flops / res[k + "_ms"] / 1e6, 1)
     res["graph_speedup_vs_separate"] = round(res["separate_default_ms"] / res["graph_ms"], 3)
+    # every matrix's A, B and C once, over the replay's whole time
+    res["algorithmic_bytes"] = alg
+    res["graph_roofline_frac"] = round(alg / (res["graph_ms"] * 1e-3) / HBM_PEAK, 4)
     log("batched: %s" % res)
     batch.close()
     for M in mats:
