@@ -95,6 +95,42 @@ def test_nonfinite_propagation(swift, cuda, name, K, form):
     M.close()
 
 
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "banded_3000"])
+@pytest.mark.parametrize("K", [32, 128])
+def test_f16_subnormal_products(swift, cuda, name, K):
+    """F16 storage keeps B values that scale into the F16 subnormal range.
+    Column k of B holds one value of 1 and otherwise values near 2^-30, which
+    its per-column scale (max / scale <= 2^14) maps to about 2^-16, below the
+    smallest normal F16 (2^-14).  A's values are multiples of 1/8 in [1/8, 1],
+    exact in F16 at any power-of-two scale.  The product must equal an FP64 sum
+    of the same F16-rounded operands to 1e-5 of (|A||B|)[i,k]: a walk that
+    flushed subnormal products to zero would miss whole outputs."""
+    import torch
+    import scipy.sparse as sp
+    m, n, rp, ci, _ = CASES[name]
+    rng = np.random.default_rng(11)
+    vals = (rng.integers(1, 9, len(ci)) / 8.0 * rng.choice([-1, 1], len(ci))).astype(np.float32)
+    B = (rng.uniform(0.5, 1.0, (n, K)) * 2.0 ** -30).astype(np.float32)
+    B *= rng.choice([-1, 1], (n, K)).astype(np.float32)
+    B[rng.integers(n, size=K), np.arange(K)] = 1.0  # one large value per column
+    M = swift.SwiftMatrix(m, n, rp, ci, vals, dtype=torch.float32)
+    M.prepare_values(swift.F16)
+    Bt = torch.tensor(B, device="cuda")
+    Bh, sc = swift.convert_dense(Bt, swift.F16, col_scale=True)
+    C = M.spmm_scaled(Bh, sc, layout=swift.ROW_MAJOR).cpu().numpy()
+    scale = sc.cpu().numpy().astype(np.float64)
+    Bq = (B / scale[None, :]).astype(np.float16).astype(np.float64) * scale[None, :]
+    assert np.abs(Bh.cpu().numpy().astype(np.float64) * scale[None, :] - Bq).max() == 0.0
+    assert (np.abs(Bq / scale[None, :]) < 2.0 ** -14).mean() > 0.9  # mostly subnormal
+    A = sp.csr_matrix((vals.astype(np.float64), np.asarray(ci), np.asarray(rp)), shape=(m, n))
+    R = A @ Bq
+    absR = abs(A) @ np.abs(Bq)
+    err = float(np.max(np.abs(C - R) / (absR + 1e-300)))
+    print("f16 subnormal", name, K, "err", err, flush=True)
+    assert err <= 1e-5
+    M.close()
+
+
 @pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "banded_3000"])
 @pytest.mark.parametrize("K", [32, 128])
 @pytest.mark.parametrize("layout", ["ROW", "COL", "ROW_FLAT", "ROW_WIN"])
