@@ -485,6 +485,15 @@ __global__ void k_empty_compact(int m, const int *__restrict__ flag, const int *
     if (r < m && flag[r]) out[idx[r]] = r;
 }
 
+__global__ void k_mark_rows(const int *__restrict__ rows, int n, int *__restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[rows[i]] = 1;  // equal values: concurrent writes agree
+}
+__global__ void k_flag_bytes(int m, const int *__restrict__ flag, unsigned char *__restrict__ out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m) out[r] = flag[r] ? 1 : 0;
+}
+
 __global__ void k_inv_perm(const int *__restrict__ perm, int n, int *__restrict__ inv) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) inv[perm[i]] = i;
@@ -604,7 +613,7 @@ static void free_matrix_arrays(swift_matrix *M) {
                     M->rowsorted, M->rowsort_src, M->csr_rowptr, M->csr_colidx,
                     M->sortval_lp, M->reside_val_lp, M->work, M->rv_ptr, M->rv_col,
                     M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->seg, M->split_rows,
-                    M->rv_row, M->empty_rows};
+                    M->rv_row, M->empty_rows, M->resrow_flag, M->reside_rows};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : M->cls) {
@@ -991,6 +1000,28 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
             SWIFT_CHECK_LAUNCH();
         }
         M->nempty = ne;
+        // rows that receive reside entries (flags and list)
+        SWIFT_TRY(fill_i32(flag, 0, (size_t)m + 1, s));
+        if (M->reside_nnz) {
+            hipLaunchKernelGGL(k_mark_rows, dim3(grid1(M->reside_nnz)), dim3(256), 0, s,
+                               M->reside_cscrowidx, M->reside_nnz, flag);
+            SWIFT_CHECK_LAUNCH();
+        }
+        SWIFT_TRY(dalloc(&M->resrow_flag, (size_t)m));
+        hipLaunchKernelGGL(k_flag_bytes, dim3(grid1(m)), dim3(256), 0, s, m, flag,
+                           M->resrow_flag);
+        SWIFT_CHECK_LAUNCH();
+        SWIFT_TRY(exclusive_scan(flag, idx.as<int>(), (size_t)m, idx.as<int>() + m, tmp.ptr, s));
+        int nr = 0;
+        SWIFT_HIP(hipMemcpyAsync(&nr, idx.as<int>() + m, sizeof(int), hipMemcpyDeviceToHost, s));
+        SWIFT_HIP(hipStreamSynchronize(s));
+        SWIFT_TRY(dalloc(&M->reside_rows, (size_t)nr));
+        if (nr) {
+            hipLaunchKernelGGL(k_empty_compact, dim3(grid1(m)), dim3(256), 0, s, m, flag,
+                               idx.as<int>(), M->reside_rows);
+            SWIFT_CHECK_LAUNCH();
+        }
+        M->nreside_rows = nr;
     }
     SWIFT_HIP(hipStreamSynchronize(s));
     SWIFT_TRY(build_segments(M, 128, s));

@@ -242,6 +242,11 @@ struct RowArgs {
     // buffer-resource path (row-view walk): bytes of B from its K-slice start,
     // and the row stride in bytes
     unsigned bbytes, bstride;
+    // row-view walk with the reside blocks in the same launch: rows flagged
+    // here are stored with atomics (nullptr: no row is); blocks from
+    // walk_blocks on run the reside blocks
+    const unsigned char *rflag;
+    int walk_blocks;
 };
 
 // B can be read through a buffer resource with 24-bit row offsets: fewer than
@@ -535,9 +540,11 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
     const TB *B = static_cast<const TB *>(a.B);
     __amdgpu_buffer_rsrc_t rsrc;
     if constexpr (BUF) rsrc = make_rsrc(B + a.kbase, a.bbytes);
-    // staged row ends: lane l holds the end of row rbase + l
+    // staged row ends: lane l holds the end of row rbase + l (and its flag of
+    // reside entries)
     int rbase = row;
     int rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
+    int rfl_l = (a.rflag && rbase + lane < a.m) ? (int)a.rflag[rbase + lane] : 0;
     // staged entries: lane l holds entry ebase + l
     int ebase = e, mc = 0;
     TC mv = (TC)0;
@@ -559,6 +566,7 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
         if (row - rbase >= 64) {
             rbase = row;
             rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
+            rfl_l = (a.rflag && rbase + lane < a.m) ? (int)a.rflag[rbase + lane] : 0;
         }
         // wave-uniform: kept in scalar registers so the slot tests below branch
         const int rend = __builtin_amdgcn_readfirstlane(__shfl(rend_l, row - rbase, 64));
@@ -684,9 +692,10 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
             for (int q = 0; q < Q; q++)
 #pragma unroll
                 for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
-        if (h == 0)
-            put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, last || (row == st.x && carried_in), acc,
-                                           HOIST ? osc : nullptr);
+        // rows that other waves (or reside blocks) also add into take atomics
+        const bool shared = last || (row == st.x && carried_in) ||
+                            (a.rflag && __builtin_amdgcn_readlane(rfl_l, row - rbase));
+        if (h == 0) put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, shared, acc, HOIST ? osc : nullptr);
         else
 #pragma unroll
             for (int q = 0; q < Q; q++)

@@ -78,7 +78,14 @@ __global__ __launch_bounds__(256) void k_tiles_row(RowArgs a) {
 // Thin kernels over the device functions of rowview.h.
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF>
 __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
-    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    if (blockIdx.x >= (unsigned)a.walk_blocks) {
+        // the reside blocks, one wave each, in the same launch (their rows
+        // are stored with atomics by the walk)
+        const int blk = (int)(((blockIdx.x - a.walk_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
+        if (blk < a.nblk) reside_block<TV, TB, TC, VEC, Q, LPE, MASK>(a, blk);
+        return;
+    }
+    const unsigned lb = xcd_block(blockIdx.x, (unsigned)a.walk_blocks);
     // wave-uniform segment id, kept in a scalar register
     const int sid = __builtin_amdgcn_readfirstlane((int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6));
     if (sid >= a.nseg) return;
@@ -241,13 +248,16 @@ __global__ __launch_bounds__(256) void k_flat_row(RowArgs a) {
     flat_walk<TV, TB, TC, VEC, Q, LPE, MASK, U>(a, t / LPE);
 }
 
-// Zeroes C[row, kbase : kbase+kcount) of each split row; one wave per row.
+// Zeroes C[row, 0 : K) of each listed row (rows, then rows2); one wave per row.
 template <class TC>
 __global__ __launch_bounds__(256) void k_zero_rows(const int *__restrict__ rows, int nrows,
-                                                   TC *__restrict__ C, int ldc, int K) {
+                                                   TC *__restrict__ C, int ldc, int K,
+                                                   const int *__restrict__ rows2 = nullptr,
+                                                   int nrows2 = 0) {
     const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (i >= nrows) return;
-    TC *c = C + (size_t)rows[i] * ldc;
+    if (i >= nrows + nrows2) return;
+    const int r = i < nrows ? rows[i] : rows2[i - nrows];
+    TC *c = C + (size_t)r * ldc;
     for (int k = threadIdx.x & 63; k < K; k += 64) c[k] = (TC)0;
 }
 
@@ -483,14 +493,18 @@ static void launch_row_shape(const RowArgs &a, RowKind kind, hipStream_t s) {
         return;
     }
     if (kind == ROW_VIEW) {
-        if (!a.nseg) return;
-        const dim3 grid(ceil_div((size_t)a.nseg * 64, 256));
+        // a.rflag set: the reside blocks run as extra blocks of this launch
+        RowArgs w = a;
+        w.walk_blocks = (int)ceil_div((size_t)a.nseg * 64, 256);
+        const size_t res = a.rflag ? ceil_div((size_t)a.nblk * 64, 256) : 0;
+        const dim3 grid((unsigned)(w.walk_blocks + res));
+        if (!grid.x) return;
         if (a.bstride)
             hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, true>), grid,
-                               dim3(256), 0, s, a);
+                               dim3(256), 0, s, w);
         else
             hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, false>), grid,
-                               dim3(256), 0, s, a);
+                               dim3(256), 0, s, w);
         return;
     }
     const unsigned items = kind == ROW_TILES ? (unsigned)a.nwork : (unsigned)a.nblk;
@@ -579,10 +593,18 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     const bool winw = kernel == SWIFT_KERNEL_WINDOW;
     const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW || flat ||
                       winw;
+    // DEFAULT: the reside blocks join the walk's launch; the rows they add
+    // into are zeroed with the split rows and stored with atomics by the walk.
+    // ROWVIEW keeps them in a launch of their own (the reference's phases).
+    const bool fused = kernel == SWIFT_KERNEL_DEFAULT && a.nblk > 0 && M->resrow_flag;
+    const int nres_rows = fused ? M->nreside_rows : 0;
+    a.rflag = fused ? M->resrow_flag : nullptr;
     if (view) {
-        if (!a.accumulate && nsplit) {
-            hipLaunchKernelGGL(k_zero_rows<TC>, dim3(ceil_div((size_t)nsplit * 64, 256)),
-                               dim3(256), 0, s, split_rows, nsplit, (TC *)C, K, K);
+        if (!a.accumulate && nsplit + nres_rows) {
+            hipLaunchKernelGGL(k_zero_rows<TC>,
+                               dim3(ceil_div((size_t)(nsplit + nres_rows) * 64, 256)), dim3(256),
+                               0, s, split_rows, nsplit, (TC *)C, K, K,
+                               (const int *)M->reside_rows, nres_rows);
             SWIFT_CHECK_LAUNCH();
         }
         if (flat && !a.accumulate && M->nempty) {
@@ -606,6 +628,7 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     }
     mark(2, s);
     for (auto &sl : slices) {
+        if (fused) break;
         a.kbase = sl.first;
         const RowTarget &t = sl.second;
         a.kcount = t.mask ? std::min(64, K - sl.first) : t.vec * t.lpe * t.q;

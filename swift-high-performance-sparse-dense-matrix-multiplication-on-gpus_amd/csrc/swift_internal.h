@@ -175,6 +175,12 @@ struct swift_matrix {
     int nempty = 0;
     void *rv_val_lp = nullptr;   // BF16/F16 copy (swift_prepare_values)
     int *inv_perm = nullptr;     // inv_perm[perm[i]] = i
+    // rows that receive reside entries: the default walk stores them with
+    // atomics after zeroing them with the split rows, so the reside blocks can
+    // run inside the walk's launch
+    unsigned char *resrow_flag = nullptr;  // m bytes, 1 = row has reside entries
+    int *reside_rows = nullptr;            // those rows, ascending
+    int nreside_rows = 0;
     // merge-path split of (rows + regular entries) into segments of mp_items
     int mp_items = 0, nseg = 0, nsplit = 0;
     int2 *seg = nullptr;         // nseg+1 (row, entry) segment starts
