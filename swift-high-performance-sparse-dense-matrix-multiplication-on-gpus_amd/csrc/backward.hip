@@ -170,6 +170,9 @@ struct BwArgs {
     const int *split_rows;
     int nsplit;
     unsigned gbytes, gstride;  // buffer-resource path for G (0: plain loads)
+    // dvals alone: the reside blocks run as blocks walk_blocks.. of the
+    // walk's launch
+    int walk_blocks;
 };
 
 template <class T> struct V16;
@@ -339,6 +342,56 @@ __device__ __forceinline__ void bw_pick(int nu, const BwArgs &a, __amdgpu_buffer
     }
 }
 
+// Reside blocks of the walked matrix: column operand G[c] loaded once, each
+// entry gathers H[row]; dvals = <H[row], G[c]>, dB[row] += v * G[c].
+template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
+__device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
+    constexpr int EPS = 64 / LPE;
+    const int lane = threadIdx.x & 63;
+    const int h = lane / LPE, sub = lane % LPE;
+    const unsigned u = a.colorblk[blk];
+    const int r = (int)(u & 0x7FFFFFFFu);
+    const bool split = (u >> 31) != 0;
+    const int start = split ? a.blkstart[blk] : a.rptr[r];
+    const int stop = split ? a.blkstop[blk] : a.rptr[r + 1];
+    const int c = a.perm[a.n - 1 - r];
+    const T *G = static_cast<const T *>(a.G);
+    const T *H = static_cast<const T *>(a.H);
+    const T *rval = static_cast<const T *>(a.rval);
+    // lane-strided layout k = q*LPE*VEC + x*LPE + sub: for a fixed (q, x) the
+    // LPE lanes touch LPE consecutive elements, so each dB atomic instruction
+    // covers whole cache lines
+    T gc[Q][VEC];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int x = 0; x < VEC; x++) gc[q][x] = G[(size_t)c * a.K + q * LPE * VEC + x * LPE + sub];
+    for (int e0 = start; e0 < stop; e0 += EPS) {
+        const int e = e0 + h;
+        const bool ok = e < stop;
+        const int row = ok ? a.rrow[e] : 0;
+        if (DV) {
+            T d = (T)0;
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int x = 0; x < VEC; x++)
+                    if (ok) d += H[(size_t)row * a.K + q * LPE * VEC + x * LPE + sub] * gc[q][x];
+            d = group_sum<T, LPE>(d);
+            if (ok && sub == 0) static_cast<T *>(a.dvals)[a.rsrc[e]] = d;
+        }
+        if (DB && ok) {
+            const T v = rval[e];
+            T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K;
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+#pragma unroll
+                for (int x = 0; x < VEC; x++)
+                    atomic_add_bw(drow + q * LPE * VEC + x * LPE + sub, v * gc[q][x]);
+        }
+    }
+}
+
 // Slots per step of the backward walk.  8 for 32/64-lane groups (as in the
 // forward walk) was measured slower on the 1M-row matrix (fused 1.06 -> 1.30
 // ms at K=128), so every shape takes 4.
@@ -347,12 +400,20 @@ __device__ __forceinline__ void bw_pick(int nu, const BwArgs &a, __amdgpu_buffer
 #endif
 template <class T, int VEC, int Q, int LPE, bool DB, bool DV, bool BUF>
 __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
+    if constexpr (!DB) {
+        if (blockIdx.x >= (unsigned)a.walk_blocks) {
+            const int blk =
+                (int)(((blockIdx.x - a.walk_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
+            if (blk < a.nblk) bw_reside_block<T, VEC, Q, LPE, DB, DV>(a, blk);
+            return;
+        }
+    }
     constexpr int EPS = 64 / LPE;
     constexpr int UNROLL = (LPE >= 32 && Q <= 2) ? SWIFT_BW_UNROLL_LONG : 4;
     constexpr int STEP = EPS * UNROLL;
     const int lane = threadIdx.x & 63;
     const int h = lane / LPE, sub = lane % LPE;
-    const unsigned lb = xcd_block_bw(blockIdx.x, gridDim.x);
+    const unsigned lb = xcd_block_bw(blockIdx.x, (unsigned)a.walk_blocks);
     const int sid = __builtin_amdgcn_readfirstlane((int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6));
     if (sid >= a.nseg) return;
     const int2 st = a.seg[sid], en = a.seg[sid + 1];
@@ -500,58 +561,6 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     }
 }
 
-// Reside blocks of the walked matrix: column operand G[c] loaded once, each
-// entry gathers H[row]; dvals = <H[row], G[c]>, dB[row] += v * G[c].
-template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
-__global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
-    constexpr int EPS = 64 / LPE;
-    const int lane = threadIdx.x & 63;
-    const int h = lane / LPE, sub = lane % LPE;
-    const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (blk >= a.nblk) return;
-    const unsigned u = a.colorblk[blk];
-    const int r = (int)(u & 0x7FFFFFFFu);
-    const bool split = (u >> 31) != 0;
-    const int start = split ? a.blkstart[blk] : a.rptr[r];
-    const int stop = split ? a.blkstop[blk] : a.rptr[r + 1];
-    const int c = a.perm[a.n - 1 - r];
-    const T *G = static_cast<const T *>(a.G);
-    const T *H = static_cast<const T *>(a.H);
-    const T *rval = static_cast<const T *>(a.rval);
-    // lane-strided layout k = q*LPE*VEC + x*LPE + sub: for a fixed (q, x) the
-    // LPE lanes touch LPE consecutive elements, so each dB atomic instruction
-    // covers whole cache lines
-    T gc[Q][VEC];
-#pragma unroll
-    for (int q = 0; q < Q; q++)
-#pragma unroll
-        for (int x = 0; x < VEC; x++) gc[q][x] = G[(size_t)c * a.K + q * LPE * VEC + x * LPE + sub];
-    for (int e0 = start; e0 < stop; e0 += EPS) {
-        const int e = e0 + h;
-        const bool ok = e < stop;
-        const int row = ok ? a.rrow[e] : 0;
-        if (DV) {
-            T d = (T)0;
-#pragma unroll
-            for (int q = 0; q < Q; q++)
-#pragma unroll
-                for (int x = 0; x < VEC; x++)
-                    if (ok) d += H[(size_t)row * a.K + q * LPE * VEC + x * LPE + sub] * gc[q][x];
-            d = group_sum<T, LPE>(d);
-            if (ok && sub == 0) static_cast<T *>(a.dvals)[a.rsrc[e]] = d;
-        }
-        if (DB && ok) {
-            const T v = rval[e];
-            T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K;
-#pragma unroll
-            for (int q = 0; q < Q; q++)
-#pragma unroll
-                for (int x = 0; x < VEC; x++)
-                    atomic_add_bw(drow + q * LPE * VEC + x * LPE + sub, v * gc[q][x]);
-        }
-    }
-}
-
 template <class T>
 __global__ void k_zero_rows_bw(const int *__restrict__ rows, int nrows, T *__restrict__ D, int K) {
     const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
@@ -561,21 +570,38 @@ __global__ void k_zero_rows_bw(const int *__restrict__ rows, int nrows, T *__res
 }
 
 template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
-static int launch_bw(const BwArgs &a, hipStream_t s) {
+__global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
+    const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (blk < a.nblk) bw_reside_block<T, VEC, Q, LPE, DB, DV>(a, blk);
+}
+
+// dvals alone (SDDMM): the reside blocks run as extra blocks of the walk's
+// launch, since the two parts write different entries (C2 K=128: 0.947 ->
+// 0.905 ms).  With dB the reside blocks add into rows the walk stores, so they
+// run in a launch of their own after the walk.  (Flagging those rows for
+// atomics, as the forward launch does, cost more than it saved: the
+// transposed 1M-row matrix has many of them, fused backward 1.06 -> 1.13 ms,
+// and the flag took the 8-lane walk from 6 to 5 waves per SIMD.)
+template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
+static int launch_bw(const BwArgs &a0, hipStream_t s) {
+    BwArgs a = a0;
+    constexpr bool fuse = !DB;
     if (DB && a.nsplit)
         hipLaunchKernelGGL(k_zero_rows_bw<T>, dim3(ceil_div((size_t)a.nsplit * 64, 256)), dim3(256),
                            0, s, a.split_rows, a.nsplit, (T *)a.dB, a.K);
-    if (a.nseg) {
-        const dim3 grid(ceil_div((size_t)a.nseg * 64, 256));
+    a.walk_blocks = (int)ceil_div((size_t)a.nseg * 64, 256);
+    const size_t res_blocks = ceil_div((size_t)a.nblk * 64, 256);
+    const dim3 grid((unsigned)(a.walk_blocks + (fuse ? res_blocks : 0)));
+    if (grid.x) {
         if (a.gstride)
             hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, DB, DV, true>), grid, dim3(256), 0, s, a);
         else
             hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, DB, DV, false>), grid, dim3(256), 0, s,
                                a);
     }
-    if (a.nblk)
-        hipLaunchKernelGGL((k_bw_reside<T, VEC, Q, LPE, DB, DV>),
-                           dim3(ceil_div((size_t)a.nblk * 64, 256)), dim3(256), 0, s, a);
+    if (!fuse && res_blocks)
+        hipLaunchKernelGGL((k_bw_reside<T, VEC, Q, LPE, DB, DV>), dim3(res_blocks), dim3(256), 0,
+                           s, a);
     SWIFT_CHECK_LAUNCH();
     return SWIFT_OK;
 }

@@ -243,7 +243,7 @@ struct RowArgs {
     // and the row stride in bytes
     unsigned bbytes, bstride;
     // row-view walk with the reside blocks in the same launch: rows flagged
-    // here are stored with atomics (nullptr: no row is); blocks from
+    // here are stored with atomics (nullptr: separate launches); blocks from
     // walk_blocks on run the reside blocks
     const unsigned char *rflag;
     int walk_blocks;
@@ -516,8 +516,11 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 // WIN: the workgroup has staged a window of B rows in LDS (win).  A step
 // whose used slots all fall inside the window reads them from LDS; any other
 // step reads B from memory as without a window.
+//
+// FLAGS: a.rflag marks rows that reside blocks of the same launch also add
+// into; the walk stores them with atomics.
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF,
-          bool WIN = false>
+          bool WIN = false, bool FLAGS = false>
 __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window win = Window{}) {
     static_assert(!WIN || (BUF && !MASK), "the window walk is a buffer-load, unmasked walk");
     constexpr int EPS = 64 / LPE;
@@ -544,7 +547,8 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
     // reside entries)
     int rbase = row;
     int rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
-    int rfl_l = (a.rflag && rbase + lane < a.m) ? (int)a.rflag[rbase + lane] : 0;
+    int rfl_l = 0;
+    if constexpr (FLAGS) rfl_l = (rbase + lane < a.m) ? (int)a.rflag[rbase + lane] : 0;
     // staged entries: lane l holds entry ebase + l
     int ebase = e, mc = 0;
     TC mv = (TC)0;
@@ -566,7 +570,7 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
         if (row - rbase >= 64) {
             rbase = row;
             rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
-            rfl_l = (a.rflag && rbase + lane < a.m) ? (int)a.rflag[rbase + lane] : 0;
+            if constexpr (FLAGS) rfl_l = (rbase + lane < a.m) ? (int)a.rflag[rbase + lane] : 0;
         }
         // wave-uniform: kept in scalar registers so the slot tests below branch
         const int rend = __builtin_amdgcn_readfirstlane(__shfl(rend_l, row - rbase, 64));
@@ -693,8 +697,8 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
 #pragma unroll
                 for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
         // rows that other waves (or reside blocks) also add into take atomics
-        const bool shared = last || (row == st.x && carried_in) ||
-                            (a.rflag && __builtin_amdgcn_readlane(rfl_l, row - rbase));
+        bool shared = last || (row == st.x && carried_in);
+        if constexpr (FLAGS) shared = shared || __builtin_amdgcn_readlane(rfl_l, row - rbase);
         if (h == 0) put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, shared, acc, HOIST ? osc : nullptr);
         else
 #pragma unroll

@@ -76,21 +76,29 @@ __global__ __launch_bounds__(256) void k_tiles_row(RowArgs a) {
 }
 
 // Thin kernels over the device functions of rowview.h.
-template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF>
+// FUSE: blocks from a.walk_blocks on run the reside blocks, one wave each,
+// and the walk stores the rows they add into with atomics.
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF,
+          bool FUSE = false>
 __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
-    if (blockIdx.x >= (unsigned)a.walk_blocks) {
-        // the reside blocks, one wave each, in the same launch (their rows
-        // are stored with atomics by the walk)
-        const int blk = (int)(((blockIdx.x - a.walk_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
-        if (blk < a.nblk) reside_block<TV, TB, TC, VEC, Q, LPE, MASK>(a, blk);
-        return;
+    if constexpr (FUSE) {
+        if (blockIdx.x >= (unsigned)a.walk_blocks) {
+            const int blk =
+                (int)(((blockIdx.x - a.walk_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
+            if (blk < a.nblk) reside_block<TV, TB, TC, VEC, Q, LPE, MASK>(a, blk);
+            return;
+        }
     }
-    const unsigned lb = xcd_block(blockIdx.x, (unsigned)a.walk_blocks);
+    const unsigned lb = xcd_block(blockIdx.x, FUSE ? (unsigned)a.walk_blocks : gridDim.x);
     // wave-uniform segment id, kept in a scalar register
     const int sid = __builtin_amdgcn_readfirstlane((int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6));
     if (sid >= a.nseg) return;
-    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF>(a, sid);
+    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF, false, FUSE>(a, sid);
 }
+
+// Lane shapes whose walk keeps its register count with the reside blocks and
+// the row flags in the same kernel (wider shapes would lose waves per SIMD).
+constexpr bool fusable_shape(int q, int lpe, bool mask) { return !mask && q == 1 && lpe <= 32; }
 
 // Window walk.  One workgroup of WIN_WAVES waves takes WIN_WAVES consecutive
 // segments (one per wave) and first picks the B rows its entries reuse most:
@@ -493,12 +501,23 @@ static void launch_row_shape(const RowArgs &a, RowKind kind, hipStream_t s) {
         return;
     }
     if (kind == ROW_VIEW) {
-        // a.rflag set: the reside blocks run as extra blocks of this launch
         RowArgs w = a;
         w.walk_blocks = (int)ceil_div((size_t)a.nseg * 64, 256);
-        const size_t res = a.rflag ? ceil_div((size_t)a.nblk * 64, 256) : 0;
-        const dim3 grid((unsigned)(w.walk_blocks + res));
-        if (!grid.x) return;
+        if constexpr (fusable_shape(Q, LPE, MASK)) {
+            if (a.rflag) {
+                // the reside blocks run as extra blocks of this launch
+                const dim3 grid((unsigned)(w.walk_blocks + ceil_div((size_t)a.nblk * 64, 256)));
+                if (a.bstride)
+                    hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, true, true>),
+                                       grid, dim3(256), 0, s, w);
+                else
+                    hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, false, true>),
+                                       grid, dim3(256), 0, s, w);
+                return;
+            }
+        }
+        if (!a.nseg) return;
+        const dim3 grid((unsigned)w.walk_blocks);
         if (a.bstride)
             hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, true>), grid,
                                dim3(256), 0, s, w);
@@ -593,10 +612,16 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     const bool winw = kernel == SWIFT_KERNEL_WINDOW;
     const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW || flat ||
                       winw;
-    // DEFAULT: the reside blocks join the walk's launch; the rows they add
-    // into are zeroed with the split rows and stored with atomics by the walk.
-    // ROWVIEW keeps them in a launch of their own (the reference's phases).
-    const bool fused = kernel == SWIFT_KERNEL_DEFAULT && a.nblk > 0 && M->resrow_flag;
+    // DEFAULT: the reside blocks join the walk's launch when few rows (<= 1 in
+    // 64) receive reside entries and every K slice has a lane shape that keeps
+    // its register count (fusable_shape); those rows are zeroed with the split
+    // rows and stored with atomics by the walk.  With many such rows the
+    // atomics would cost the walk more than the launch saves.  ROWVIEW keeps
+    // the reside blocks in a launch of their own (the reference's phases).
+    bool fusable = true;
+    for (auto &sl : slices) fusable = fusable && fusable_shape(sl.second.q, sl.second.lpe, sl.second.mask);
+    const bool fused = kernel == SWIFT_KERNEL_DEFAULT && fusable && a.nblk > 0 &&
+                       M->resrow_flag && M->nreside_rows <= M->m / 64;
     const int nres_rows = fused ? M->nreside_rows : 0;
     a.rflag = fused ? M->resrow_flag : nullptr;
     if (view) {
