@@ -781,7 +781,7 @@ static int spmm_col(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
         TB *Bt = M->scratch.as<TB>();
         TC *Ct = reinterpret_cast<TC *>(static_cast<char *>(M->scratch.ptr) + bB);
         SWIFT_TRY(transpose<TB>(static_cast<const TB *>(B), Bt, K, M->n, false, s));
-        SWIFT_TRY((spmm_row<TV, TB, TC>(M, K, SWIFT_KERNEL_ROWVIEW, flags & ~SWIFT_ACCUMULATE, Bt,
+        SWIFT_TRY((spmm_row<TV, TB, TC>(M, K, SWIFT_KERNEL_DEFAULT, flags & ~SWIFT_ACCUMULATE, Bt,
                                         Ct, sval, rval, rv_val, kscale, gscale, s)));
         return transpose<TC>(Ct, static_cast<TC *>(C), M->m, K, accumulate, s);
     }
