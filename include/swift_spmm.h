@@ -254,7 +254,11 @@ int swift_spmm(const swift_matrix *mat, int K, swift_layout layout,
 /* swift_spmm with HIP events recorded on `stream` between its launches, the
  * way the reference times only its kernels (src/Swift.h:298-336, where C is
  * zeroed before the timed region): kernel_ms[0] = zeroing of C, [1] = regular
- * (tile) kernels, [2] = irregular (reside) kernels.  Synchronises `stream`. */
+ * (tile) kernels, [2] = irregular (reside) kernels.  SWIFT_KERNEL_DEFAULT
+ * row-major may run the reside blocks inside the regular launch (when few rows
+ * receive reside entries); [1] then includes them and [2] is only the gap
+ * between two events.  SWIFT_KERNEL_ROWVIEW always keeps them apart.
+ * Synchronises `stream`. */
 int swift_spmm_timed(const swift_matrix *mat, int K, swift_layout layout,
                      swift_dtype dtype, swift_kernel kernel, unsigned flags,
                      const void *d_B, void *d_C, swift_stream_t stream,
