@@ -132,7 +132,7 @@ def test_f16_subnormal_products(swift, cuda, name, K):
 
 
 @pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "banded_3000"])
-@pytest.mark.parametrize("K", [32, 128])
+@pytest.mark.parametrize("K", [32, 100, 128])
 @pytest.mark.parametrize("layout", ["ROW", "COL", "ROW_FLAT", "ROW_WIN"])
 def test_f16_scaled_storage(swift, oracle, cuda, name, K, layout):
     import torch
