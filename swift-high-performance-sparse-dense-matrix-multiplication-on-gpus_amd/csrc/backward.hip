@@ -73,9 +73,9 @@ static int update_values_t(swift_matrix *M, const T *csr, hipStream_t s) {
     if (M->nnz)
         hipLaunchKernelGGL(k_gather_vals<T>, dim3(g1(M->nnz)), dim3(256), 0, s, M->sortsrc_tmp, csr,
                            (size_t)M->nnz, (T *)M->sortval_tmp);
-    if (M->regular_nnz)
-        hipLaunchKernelGGL(k_gather_vals<T>, dim3(g1(M->regular_nnz)), dim3(256), 0, s, M->rv_src,
-                           csr, (size_t)M->regular_nnz, (T *)M->rv_val);
+    if (M->rv_nnz)
+        hipLaunchKernelGGL(k_gather_vals<T>, dim3(g1(M->rv_nnz)), dim3(256), 0, s, M->rv_src,
+                           csr, (size_t)M->rv_nnz, (T *)M->rv_val);
     SWIFT_CHECK_LAUNCH();
     return SWIFT_OK;
 }
@@ -115,6 +115,7 @@ int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t 
     T->nnz = A->nnz;
     T->vtype = A->vtype;
     T->device = A->device;
+    T->rv_all = true;  // the backward walks cover the reside entries too
     int rc = build_impl(T, cscptr.as<int>(), cscrow.as<int>(), cscval.ptr, s);
     if (rc != SWIFT_OK) {
         destroy_matrix(T);
@@ -130,9 +131,9 @@ int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t 
     if (T->nnz)
         hipLaunchKernelGGL(k_compose, dim3(g1(T->nnz)), dim3(256), 0, s, T->sortsrc_tmp,
                            cscsrc.as<int>(), (size_t)T->nnz);
-    if (T->regular_nnz)
-        hipLaunchKernelGGL(k_compose, dim3(g1(T->regular_nnz)), dim3(256), 0, s, T->rv_src,
-                           cscsrc.as<int>(), (size_t)T->regular_nnz);
+    if (T->rv_nnz)
+        hipLaunchKernelGGL(k_compose, dim3(g1(T->rv_nnz)), dim3(256), 0, s, T->rv_src,
+                           cscsrc.as<int>(), (size_t)T->rv_nnz);
     SWIFT_CHECK_LAUNCH();
     SWIFT_HIP(hipStreamSynchronize(s));
     *out = T;
@@ -637,7 +638,7 @@ static BwArgs walk_args(const swift_matrix *W, int K) {
     a.rv_src = W->rv_src;
     a.rv_val = W->rv_val;
     a.n = W->n;
-    a.nblk = W->numblk;
+    a.nblk = W->rv_all ? 0 : W->numblk;  // rv_all: the walk holds the reside entries
     a.colorblk = W->colorblk;
     a.blkstart = W->blkstart;
     a.blkstop = W->blkstop;

@@ -311,7 +311,8 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
                     total += (double)b->W[i] * b->Ks[i];
                     work.push_back(total);
                 }
-                for (int q = 0; q < M->numblk; q++) res.push_back(make_int2(i, q));
+                if (!M->rv_all)  // rv_all: the walk holds the reside entries
+                    for (int q = 0; q < M->numblk; q++) res.push_back(make_int2(i, q));
                 rows.resize((size_t)M->nsplit);
                 if (M->nsplit &&
                     hipMemcpy(rows.data(), M->split_rows, sizeof(int) * M->nsplit,

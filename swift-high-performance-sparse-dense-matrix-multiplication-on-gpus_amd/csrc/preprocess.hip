@@ -457,22 +457,41 @@ __global__ void k_row_hist(const int *__restrict__ rows, int n, int *__restrict_
     if (p < n) atomicAdd(&cnt[rows[p]], 1);
 }
 
-// q-th entry of the row view <- regular entry p = order[q]: its B row is the
-// original column of sorted column tile_colidx[p/32] + p%32.
-__global__ void k_rv_fill(int R, const unsigned *__restrict__ order,
+// q-th entry of the row view <- entry p = order[q].  p < R: regular entry p,
+// whose B row is the original column of sorted column tile_colidx[p/32] +
+// p%32.  p >= R (row views of every entry): reside entry p - R of reside
+// column r (rptr[r] <= p - R < rptr[r+1], at most 32 columns), sorted column
+// n-1-r.
+__global__ void k_rv_fill(int R, int RV, const unsigned *__restrict__ order,
                           const int *__restrict__ tile_colidx, const int *__restrict__ perm,
                           const int *__restrict__ srow, const void *__restrict__ sval,
-                          const int *__restrict__ ssrc, int vs, int *__restrict__ rv_col,
-                          void *__restrict__ rv_val, int *__restrict__ rv_src,
-                          int *__restrict__ rv_row) {
+                          const int *__restrict__ ssrc, int vs, int n, int rcols,
+                          const int *__restrict__ rptr, const int *__restrict__ rrow,
+                          const void *__restrict__ rval, const int *__restrict__ rsrc,
+                          int *__restrict__ rv_col, void *__restrict__ rv_val,
+                          int *__restrict__ rv_src, int *__restrict__ rv_row) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= R) return;
+    if (q >= RV) return;
     int p = (int)order[q];
-    int c = tile_colidx[p >> 5] + (p & 31);
-    rv_col[q] = perm[c];
-    copy_value(vs, rv_val, q, sval, p);
-    rv_src[q] = ssrc[p];
-    rv_row[q] = srow[p];
+    if (p < R) {
+        int c = tile_colidx[p >> 5] + (p & 31);
+        rv_col[q] = perm[c];
+        copy_value(vs, rv_val, q, sval, p);
+        rv_src[q] = ssrc[p];
+        rv_row[q] = srow[p];
+    } else {
+        const int e = p - R;
+        int lo = 0, hi = rcols;  // largest r with rptr[r] <= e
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (rptr[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        rv_col[q] = perm[n - 1 - lo];
+        copy_value(vs, rv_val, q, rval, e);
+        rv_src[q] = rsrc[e];
+        rv_row[q] = rrow[e];
+    }
 }
 // rows whose regular part is empty
 __global__ void k_empty_flag(int m, const int *__restrict__ ptr, int *__restrict__ flag) {
@@ -885,14 +904,14 @@ static int build_seg_table(swift_matrix *M, int W, hipStream_t s, int2 **seg_out
                            int **split_out, int *nsplit_out) {
     if (*seg_out) { (void)hipFree(*seg_out); *seg_out = nullptr; }
     if (*split_out) { (void)hipFree(*split_out); *split_out = nullptr; }
-    const long long items = (long long)M->m + M->regular_nnz;
+    const long long items = (long long)M->m + M->rv_nnz;
     const int nseg = (int)((items + W - 1) / W);
     *nseg_out = nseg;
     *nsplit_out = 0;
     SWIFT_TRY(dalloc(seg_out, (size_t)nseg + 1));
     int2 *seg = *seg_out;
     hipLaunchKernelGGL(k_mp_seg, dim3(grid1((size_t)nseg + 1)), dim3(256), 0, s, M->m,
-                       M->regular_nnz, W, nseg, M->rv_ptr, seg);
+                       M->rv_nnz, W, nseg, M->rv_ptr, seg);
     SWIFT_CHECK_LAUNCH();
     if (nseg == 0) return dalloc(split_out, 1);
     Buffer flag, idx, tmp;
@@ -934,7 +953,7 @@ int build_segments(swift_matrix *M, int W, hipStream_t s) {
 int build_class_segments(swift_matrix *M, hipStream_t s) {
     static const int by_bytes[SEG_CLASSES] = {64, 96, 192, 128, 256};
     const long long m = std::max(M->m, 1);
-    const int mean = (int)((m + M->regular_nnz + m - 1) / m);
+    const int mean = (int)((m + M->rv_nnz + m - 1) / m);
     for (int c = 0; c < SEG_CLASSES; c++) {
         swift_matrix::SegTable &t = M->cls[c];
         t.W = std::min(1024, std::max(by_bytes[c], 4 * mean));
@@ -943,10 +962,14 @@ int build_class_segments(swift_matrix *M, hipStream_t s) {
     return SWIFT_OK;
 }
 
-// Row view of the regular part: stable radix sort of the regular entries by
-// row, so each row keeps its entries in tile order.
+// Row view of the regular part (with rv_all, of every entry): stable radix
+// sort of the entries by row, so each row keeps its regular entries in tile
+// order, followed by its reside entries.
 int build_rowview(swift_matrix *M, hipStream_t s) {
-    const int m = M->m, R = M->regular_nnz;
+    const int m = M->m, R0 = M->regular_nnz;
+    const int NRES = M->rv_all ? M->reside_nnz : 0;
+    const int R = R0 + NRES;
+    M->rv_nnz = R;
     const int vs = vsize_of(M->vtype);
     SWIFT_TRY(dalloc(&M->rv_ptr, (size_t)m + 1));
     SWIFT_TRY(dalloc(&M->rv_col, (size_t)R));
@@ -962,9 +985,14 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
     Buffer cnt, tmp, keys, vals, kalt, valt;
     SWIFT_TRY(cnt.alloc(sizeof(int) * ((size_t)m + 1)));
     SWIFT_TRY(fill_i32(cnt.as<int>(), 0, (size_t)m + 1, s));
-    if (R) {
-        hipLaunchKernelGGL(k_row_hist, dim3(grid1(R)), dim3(256), 0, s, M->sortrowidx, R,
+    if (R0) {
+        hipLaunchKernelGGL(k_row_hist, dim3(grid1(R0)), dim3(256), 0, s, M->sortrowidx, R0,
                            cnt.as<int>());
+        SWIFT_CHECK_LAUNCH();
+    }
+    if (NRES) {
+        hipLaunchKernelGGL(k_row_hist, dim3(grid1(NRES)), dim3(256), 0, s, M->reside_cscrowidx,
+                           NRES, cnt.as<int>());
         SWIFT_CHECK_LAUNCH();
     }
     SWIFT_TRY(tmp.alloc(std::max(scan_temp_bytes((size_t)m + 1), radix_temp_bytes((size_t)R))));
@@ -973,14 +1001,28 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
         size_t b = sizeof(unsigned) * (size_t)R;
         SWIFT_TRY(keys.alloc(b)); SWIFT_TRY(vals.alloc(b));
         SWIFT_TRY(kalt.alloc(b)); SWIFT_TRY(valt.alloc(b));
-        SWIFT_TRY(radix_sort_pairs((const unsigned *)M->sortrowidx, nullptr, keys.as<unsigned>(),
-                                   vals.as<unsigned>(), kalt.as<unsigned>(), valt.as<unsigned>(),
-                                   (size_t)R, bits_for((unsigned)(m > 0 ? m - 1 : 0)), tmp.ptr,
-                                   s));
-        hipLaunchKernelGGL(k_rv_fill, dim3(grid1(R)), dim3(256), 0, s, R, vals.as<unsigned>(),
-                           M->tile_colidx, M->perm, M->sortrowidx, M->sortval, M->sortsrc, vs,
-                           M->rv_col, M->rv_val, M->rv_src, M->rv_row);
+        // sort keys: the regular entries' rows, then the reside entries' rows
+        Buffer rows;
+        const unsigned *kin = (const unsigned *)M->sortrowidx;
+        if (NRES) {
+            SWIFT_TRY(rows.alloc(b));
+            if (R0)
+                SWIFT_HIP(hipMemcpyAsync(rows.ptr, M->sortrowidx, sizeof(int) * (size_t)R0,
+                                         hipMemcpyDeviceToDevice, s));
+            SWIFT_HIP(hipMemcpyAsync(rows.as<int>() + R0, M->reside_cscrowidx,
+                                     sizeof(int) * (size_t)NRES, hipMemcpyDeviceToDevice, s));
+            kin = rows.as<unsigned>();
+        }
+        SWIFT_TRY(radix_sort_pairs(kin, nullptr, keys.as<unsigned>(), vals.as<unsigned>(),
+                                   kalt.as<unsigned>(), valt.as<unsigned>(), (size_t)R,
+                                   bits_for((unsigned)(m > 0 ? m - 1 : 0)), tmp.ptr, s));
+        hipLaunchKernelGGL(k_rv_fill, dim3(grid1(R)), dim3(256), 0, s, R0, R,
+                           vals.as<unsigned>(), M->tile_colidx, M->perm, M->sortrowidx,
+                           M->sortval, M->sortsrc, vs, M->n, M->reside_col, M->reside_cscptr,
+                           M->reside_cscrowidx, M->reside_val, M->reside_src, M->rv_col,
+                           M->rv_val, M->rv_src, M->rv_row);
         SWIFT_CHECK_LAUNCH();
+        SWIFT_HIP(hipStreamSynchronize(s));  // rows is freed at the end of this block
     }
     // rows without regular entries
     if (m) {

@@ -621,7 +621,7 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     bool fusable = true;
     for (auto &sl : slices) fusable = fusable && fusable_shape(sl.second.q, sl.second.lpe, sl.second.mask);
     const bool fused = kernel == SWIFT_KERNEL_DEFAULT && fusable && a.nblk > 0 &&
-                       M->resrow_flag && M->nreside_rows <= M->m / 64;
+                       !M->rv_all && M->resrow_flag && M->nreside_rows <= M->m / 64;
     const int nres_rows = fused ? M->nreside_rows : 0;
     a.rflag = fused ? M->resrow_flag : nullptr;
     if (view) {
@@ -653,7 +653,7 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     }
     mark(2, s);
     for (auto &sl : slices) {
-        if (fused) break;
+        if (fused || (view && M->rv_all)) break;  // rv_all: the walk holds the reside entries
         a.kbase = sl.first;
         const RowTarget &t = sl.second;
         a.kcount = t.mask ? std::min(64, K - sl.first) : t.vec * t.lpe * t.q;
@@ -1041,7 +1041,7 @@ int prepare_values_impl(swift_matrix *M, swift_dtype dtype, hipStream_t s) {
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     SWIFT_HIP(hipMalloc(&M->sortval_lp, std::max<size_t>(2, (size_t)M->regular_nnz * 2)));
     SWIFT_HIP(hipMalloc(&M->reside_val_lp, std::max<size_t>(2, (size_t)M->reside_nnz * 2)));
-    SWIFT_HIP(hipMalloc(&M->rv_val_lp, std::max<size_t>(2, (size_t)M->regular_nnz * 2)));
+    SWIFT_HIP(hipMalloc(&M->rv_val_lp, std::max<size_t>(2, (size_t)M->rv_nnz * 2)));
     float scale = 1.0f;
     if (dtype == SWIFT_F16) {
         Buffer mx;
@@ -1081,7 +1081,7 @@ int prepare_values_impl(swift_matrix *M, swift_dtype dtype, hipStream_t s) {
     };
     SWIFT_TRY(conv(M->sortval, M->sortval_lp, (size_t)M->regular_nnz));
     SWIFT_TRY(conv(M->reside_val, M->reside_val_lp, (size_t)M->reside_nnz));
-    SWIFT_TRY(conv(M->rv_val, M->rv_val_lp, (size_t)M->regular_nnz));
+    SWIFT_TRY(conv(M->rv_val, M->rv_val_lp, (size_t)M->rv_nnz));
     SWIFT_HIP(hipStreamSynchronize(s));
     M->lp_type = dtype;
     M->lp_scale = scale;

@@ -175,6 +175,12 @@ struct swift_matrix {
     int nempty = 0;
     void *rv_val_lp = nullptr;   // BF16/F16 copy (swift_prepare_values)
     int *inv_perm = nullptr;     // inv_perm[perm[i]] = i
+    // The row view holds the regular entries, or with rv_all every entry
+    // (regular, then reside, within each row).  rv_all is set for the
+    // transposed layout of the backward pass: its walks then need no reside
+    // launch, whose atomics into dB cost more than walking the few entries.
+    bool rv_all = false;
+    int rv_nnz = 0;
     // rows that receive reside entries: the default walk stores them with
     // atomics after zeroing them with the split rows, so the reside blocks can
     // run inside the walk's launch
