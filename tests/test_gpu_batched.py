@@ -68,10 +68,16 @@ def test_batch_matches_oracle_and_separate(swift, oracle, cuda):
     Bs, Bn, Cs = operands(items, 1)
     batch.spmm(Bs, Cs)
     check_all(oracle, items, Bn, Cs)
-    for (M, K, _), b, c in zip(items, Bs, Cs):
+    # Batched and separate launches sum a row in different FP32 orders (split rows
+    # end in atomics, so the order also varies between runs): on rows of ~2,600
+    # entries two orders differ by ~1.6e-4, beyond an elementwise 1e-5.  Each is
+    # within TOL of FP64 normalised by |A||B|, so they agree within 2*TOL.
+    for (M, K, (m, n, rp, ci, v)), b, bn, c in zip(items, Bs, Bn, Cs):
         ref = M.spmm(b)
         torch.cuda.synchronize()
-        assert torch.allclose(c, ref, rtol=1e-5, atol=1e-5)
+        _, absC = oracle.spmm_f64(m, n, rp, ci, v.astype(np.float64), bn.astype(np.float64))
+        err = gpu_common.rel_err(c.cpu().numpy(), ref.cpu().numpy().astype(np.float64), absC)
+        assert err <= 2 * TOL, "batched vs separate (%dx%d, K=%d): %.3g" % (m, n, K, err)
     batch.close()
     for it in items:
         it[0].close()
