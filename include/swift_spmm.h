@@ -54,10 +54,12 @@ typedef enum {
     SWIFT_KERNEL_ROWVIEW = 5,    /* row-major: regular part walked output-stationary
                                     through its row view (merge-path balanced), one
                                     row at a time per wave; reside part in balance
-                                    blocks                                           */
+                                    blocks, or in the row view when it is at most 1%
+                                    of the entries and reaches over 1 row in 64
+                                    (swift_info.rowview_nnz = nnz)                   */
     SWIFT_KERNEL_FLAT = 6,       /* row-major: row view walked by independent lane
                                     groups, several entries in flight whatever the
-                                    row lengths; reside part in balance blocks       */
+                                    row lengths; reside part as for ROWVIEW          */
     SWIFT_KERNEL_WINDOW = 7      /* row-major: ROWVIEW walk by workgroups of 16
                                     segments that first stage the B rows their
                                     entries reuse most (a column window) in LDS    */
@@ -96,6 +98,9 @@ typedef struct {
      * [1] blocking (staircase tiles, reside part, block_catgory, balance),
      * [2] kernel schedules (work list, row view, segment tables), [3] total. */
     float build_ms[4];
+    /* Entries in the row view that the row-view kernels walk (ours):
+     * regular_nnz, or nnz when the reside part is walked with them. */
+    int rowview_nnz;
 } swift_info;
 
 /* ------------------------------------------------------------------ host */
@@ -257,7 +262,9 @@ int swift_spmm(const swift_matrix *mat, int K, swift_layout layout,
  * (tile) kernels, [2] = irregular (reside) kernels.  SWIFT_KERNEL_DEFAULT
  * row-major may run the reside blocks inside the regular launch (when few rows
  * receive reside entries); [1] then includes them and [2] is only the gap
- * between two events.  SWIFT_KERNEL_ROWVIEW always keeps them apart.
+ * between two events.  SWIFT_KERNEL_ROWVIEW keeps them apart, unless the row
+ * view holds the reside entries (swift_info.rowview_nnz = nnz): then every
+ * row-view kernel walks them in [1].
  * Synchronises `stream`. */
 int swift_spmm_timed(const swift_matrix *mat, int K, swift_layout layout,
                      swift_dtype dtype, swift_kernel kernel, unsigned flags,

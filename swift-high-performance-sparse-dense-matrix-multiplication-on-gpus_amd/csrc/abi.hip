@@ -216,6 +216,7 @@ extern "C" int swift_get_info(const swift_matrix *M, swift_info *o) {
     o->rowsorted = M->rowsorted;
     o->sortsrc = M->sortsrc; o->reside_src = M->reside_src;
     for (int i = 0; i < 4; i++) o->build_ms[i] = M->build_ms[i];
+    o->rowview_nnz = M->rv_nnz;
     return SWIFT_OK;
 }
 

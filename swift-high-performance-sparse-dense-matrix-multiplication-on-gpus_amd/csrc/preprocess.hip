@@ -967,6 +967,26 @@ int build_class_segments(swift_matrix *M, hipStream_t s) {
 // order, followed by its reside entries.
 int build_rowview(swift_matrix *M, hipStream_t s) {
     const int m = M->m, R0 = M->regular_nnz;
+    // A small reside part (at most 1% of the entries) spread over many rows
+    // (more than 1 in 64) is walked with the regular entries: its balance
+    // blocks would each load a B row for few entries, and so many rows are
+    // too many to zero and store with atomics inside the walk's launch.
+    if (!M->rv_all && M->reside_nnz > 0 && m > 0 &&
+        (long long)M->reside_nnz * 100 <= (long long)M->nnz) {
+        Buffer fl, ix, st;
+        SWIFT_TRY(fl.alloc(sizeof(int) * ((size_t)m + 1)));
+        SWIFT_TRY(ix.alloc(sizeof(int) * ((size_t)m + 1)));
+        SWIFT_TRY(st.alloc(scan_temp_bytes((size_t)m + 1)));
+        SWIFT_TRY(fill_i32(fl.as<int>(), 0, (size_t)m + 1, s));
+        hipLaunchKernelGGL(k_mark_rows, dim3(grid1(M->reside_nnz)), dim3(256), 0, s,
+                           M->reside_cscrowidx, M->reside_nnz, fl.as<int>());
+        SWIFT_CHECK_LAUNCH();
+        SWIFT_TRY(exclusive_scan(fl.as<int>(), ix.as<int>(), (size_t)m, ix.as<int>() + m, st.ptr, s));
+        int nr = 0;
+        SWIFT_HIP(hipMemcpyAsync(&nr, ix.as<int>() + m, sizeof(int), hipMemcpyDeviceToHost, s));
+        SWIFT_HIP(hipStreamSynchronize(s));
+        M->rv_all = nr > m / 64;
+    }
     const int NRES = M->rv_all ? M->reside_nnz : 0;
     const int R = R0 + NRES;
     M->rv_nnz = R;

@@ -179,6 +179,8 @@ struct swift_matrix {
     // (regular, then reside, within each row).  rv_all is set for the
     // transposed layout of the backward pass: its walks then need no reside
     // launch, whose atomics into dB cost more than walking the few entries.
+    // build_rowview also sets it for a matrix whose reside part is at most 1%
+    // of its entries and reaches more than 1 row in 64.
     bool rv_all = false;
     int rv_nnz = 0;
     // rows that receive reside entries: the default walk stores them with

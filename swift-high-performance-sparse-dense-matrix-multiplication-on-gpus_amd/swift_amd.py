@@ -45,7 +45,7 @@ class Info(C.Structure):
                            "sortrowindex", "reside_cscptr", "reside_cscrowidx", "reside_val",
                            "colorblk", "blkstart", "blkstop", "blkptr", "val_position",
                            "val_offset", "rowsorted", "sortsrc", "reside_src")] + [
-        ("build_ms", C.c_float * 4)]
+        ("build_ms", C.c_float * 4), ("rowview_nnz", C.c_int)]
 
 
 class Stats(C.Structure):

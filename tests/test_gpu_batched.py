@@ -61,8 +61,8 @@ def test_batch_matches_oracle_and_separate(swift, oracle, cuda):
     print("batch of", len(items), flush=True)
     batch = swift.Batch([it[0] for it in items], [it[1] for it in items])
     p = batch.params()
-    lo = sum(-(-(it[0].info().m + it[0].info().regular_nnz) // 512) for it in items)
-    hi = sum(-(-(it[0].info().m + it[0].info().regular_nnz) // 64) for it in items)
+    lo = sum(-(-(it[0].info().m + it[0].info().rowview_nnz) // 512) for it in items)
+    hi = sum(-(-(it[0].info().m + it[0].info().rowview_nnz) // 64) for it in items)
     assert lo <= p["work_units"] <= hi
     assert p["depth"] in (64, 128, 256, 512) and p["block_threads"] == 256
     Bs, Bn, Cs = operands(items, 1)
