@@ -156,7 +156,8 @@ int swift_generate_dense(size_t count, int dist, double lo, double hi,
 /* Stable CSR -> CSC transpose on the GPU.  Replaces csr_to_csc
  * (src/gpu_csr2csc.h:51) and nnzpercolSegmentReduce (:24): rows ascending
  * inside each column, nnzpercol[j] = colptr[j+1]-colptr[j].  Values are of
- * vtype (F32/F64); d_csc_val may be NULL to skip values. */
+ * vtype (F32/F64); d_csc_val may be NULL to skip values.  The CSR is
+ * checked as in swift_build (SWIFT_ERR_ARG on a bad row pointer or column). */
 int swift_csr_to_csc(int m, int n, int nnz, swift_dtype vtype,
                      const void *d_csr_val, const int *d_csr_row_ptr,
                      const int *d_csr_col_ind, void *d_csc_val,
@@ -178,7 +179,10 @@ int swift_col_sort(int n, int nnz, int K, swift_dtype vtype,
 /* Whole preprocessing on the GPU: csr_to_csc -> ColSort -> formattransation
  * (src/coo2csc.h:137) -> block_catgory (src/block_catgory.h:1) -> balance
  * (src/balance.h:1).  Input CSR is device resident (vtype F32 or F64);
- * nothing of it is kept except a copy of the row pointer. */
+ * the handle keeps copies of its row pointer and column indices.  One read
+ * pass checks the input first: rowptr[0] = 0, rowptr[m] = nnz, rowptr
+ * non-decreasing and every column in [0, n), else SWIFT_ERR_ARG (the
+ * reference passes its loader output on unchecked, src/main.cu:79). */
 int swift_build(swift_matrix **out, int m, int n, int nnz, swift_dtype vtype,
                 const int *d_csr_row_ptr, const int *d_csr_col_ind,
                 const void *d_csr_val, swift_stream_t stream);

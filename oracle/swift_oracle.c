@@ -537,25 +537,29 @@ double oracle_swift_cpu_f32_timed(const oracle_swift *s, int K,
 
 /* ------------------------------------------------------ FP64 references */
 
-/* The naive CSR golden of src/main.cu:145-155, in double, plus |A||B|. */
+/* The naive CSR golden of src/main.cu:145-155, in double, plus |A||B|.
+ * Rows are independent and each row sums its entries in CSR order, so the
+ * result is the same bits whatever the number of threads. */
 void oracle_spmm_csr_f64(int m, int n, const int *rowptr, const int *colidx,
                          const double *val, int K, const double *B, double *C,
                          double *absC) {
     (void)n;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < m; i++) {
         double *c = C + (size_t)i * K;
         double *a = absC ? absC + (size_t)i * K : NULL;
-        for (int k = 0; k < K; k++) {
-            c[k] = 0.0;
-            if (a) a[k] = 0.0;
-        }
+        for (int k = 0; k < K; k++) c[k] = 0.0;
         for (int e = rowptr[i]; e < rowptr[i + 1]; e++) {
             const double *b = B + (size_t)colidx[e] * K;
-            double v = val[e];
-            for (int k = 0; k < K; k++) {
-                c[k] += v * b[k];
-                if (a) a[k] += (v < 0 ? -v : v) * (b[k] < 0 ? -b[k] : b[k]);
-            }
+            const double v = val[e];
+            for (int k = 0; k < K; k++) c[k] += v * b[k];
+        }
+        if (!a) continue;
+        for (int k = 0; k < K; k++) a[k] = 0.0;
+        for (int e = rowptr[i]; e < rowptr[i + 1]; e++) {
+            const double *b = B + (size_t)colidx[e] * K;
+            const double v = val[e] < 0 ? -val[e] : val[e];
+            for (int k = 0; k < K; k++) a[k] += v * (b[k] < 0 ? -b[k] : b[k]);
         }
     }
 }
@@ -576,6 +580,7 @@ void oracle_sddmm_csr_f64(int m, int n, const int *rowptr, const int *colidx,
                           int K, const double *dC, const double *B,
                           double *dvals) {
     (void)n;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < m; i++)
         for (int e = rowptr[i]; e < rowptr[i + 1]; e++) {
             const double *g = dC + (size_t)i * K;

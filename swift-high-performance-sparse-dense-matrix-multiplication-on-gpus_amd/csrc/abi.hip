@@ -125,8 +125,9 @@ extern "C" int swift_build_host(swift_matrix **out, int m, int n, int nnz, swift
         set_error("swift_build_host: bad arguments");
         return SWIFT_ERR_ARG;
     }
-    // the row pointer must describe nnz entries with valid columns: the GPU
-    // build trusts its input, so check it here once
+    // the row pointer must describe nnz entries with valid columns; checked
+    // here on the host before the copies (swift_build checks the device
+    // arrays again), so the message can name the entry
     if (csr_row_ptr[0] != 0 || csr_row_ptr[m] != nnz) {
         set_error("swift_build_host: rowptr[0]=%d rowptr[m]=%d nnz=%d", csr_row_ptr[0],
                   csr_row_ptr[m], nnz);

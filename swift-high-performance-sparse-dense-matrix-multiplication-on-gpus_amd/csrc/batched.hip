@@ -14,9 +14,10 @@
 // replays it; new B/C pointers only rewrite the descriptor table.
 //
 // Autotuner: matrices are keyed by (K, log2 mean nonzeros per row, reside
-// share bucket); for each key the largest matrix is timed with segment sizes
-// {64, 128, 256, 512} and the fastest is applied to every matrix of that key.
-// Results are cached per process.
+// share bucket, whether the row view holds the reside entries); for each key
+// the largest matrix is timed with segment sizes {64, 128, 256, 512} and the
+// fastest is applied to every matrix of that key.  Results are cached per
+// process.
 #include "rowview.h"
 
 #include <algorithm>
@@ -123,7 +124,10 @@ __global__ __launch_bounds__(256) void k_batch_reside(const RowArgs *__restrict_
 
 // ---------------------------------------------------------- autotuner --
 
-using TuneKey = std::tuple<int, int, int>;
+// (K, log2 mean nonzeros per row, reside share bucket, row view holds the
+// reside entries): matrices of one key walk the same entry sets with the
+// same lane shape.
+using TuneKey = std::tuple<int, int, int, int>;
 static std::map<TuneKey, int> g_tuned;
 static std::mutex g_tune_mu;
 
@@ -132,7 +136,7 @@ static TuneKey tune_key(const swift_matrix *M, int K) {
     const int dens = (int)std::floor(std::log2(mean + 1.0));
     const double share = M->nnz ? (double)M->reside_nnz / M->nnz : 0.0;
     const int res = share == 0.0 ? 0 : share < 0.01 ? 1 : share < 0.1 ? 2 : 3;
-    return TuneKey(K, dens, res);
+    return TuneKey(K, dens, res, M->rv_all ? 1 : 0);
 }
 
 // Times the row-view product of M for each candidate segment size on the

@@ -45,6 +45,55 @@ static unsigned grid1(size_t n, unsigned block = 256) {
     return (unsigned)(g ? g : 1);
 }
 
+// ------------------------------------------------------- input checking --
+
+// One in-bounds read pass over a device CSR before anything is scattered by
+// its indices.  Error bits: 1 rowptr[0] != 0, 2 rowptr[m] != nnz, 4 rowptr
+// decreases somewhere, 8 a column index outside [0, n).  A monotone row
+// pointer from 0 to nnz keeps every row range inside the column array.
+__global__ void k_check_csr(const int *__restrict__ rowptr, int m,
+                            const int *__restrict__ colidx, int nnz, int n,
+                            int *__restrict__ flag) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    if (t == 0) {
+        if (rowptr[0] != 0) bad |= 1;
+        if (rowptr[m] != nnz) bad |= 2;
+    }
+    for (size_t i = t; i < (size_t)m; i += stride)
+        if (rowptr[i + 1] < rowptr[i]) bad |= 4;
+    for (size_t e = t; e < (size_t)nnz; e += stride)
+        if ((unsigned)colidx[e] >= (unsigned)n) bad |= 8;
+    for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o, 64);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(flag, bad);
+}
+
+// Validates a caller's device CSR (m rows, n columns, nnz entries) on stream
+// s; returns SWIFT_ERR_ARG naming the first problem found.
+static int check_csr(const char *who, int m, int n, int nnz, const int *d_rowptr,
+                     const int *d_colidx, hipStream_t s) {
+    Buffer flag;
+    SWIFT_TRY(flag.alloc(sizeof(int)));
+    SWIFT_TRY(fill_i32(flag.as<int>(), 0, 1, s));
+    const size_t work = std::max<size_t>((size_t)m, (size_t)nnz);
+    const unsigned grid = (unsigned)std::min<size_t>(std::max<size_t>(1, (work + 255) / 256), 4096);
+    hipLaunchKernelGGL(k_check_csr, dim3(grid), dim3(256), 0, s, d_rowptr, m, d_colidx, nnz, n,
+                       flag.as<int>());
+    SWIFT_CHECK_LAUNCH();
+    int bad = 0;
+    SWIFT_HIP(hipMemcpyAsync(&bad, flag.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    SWIFT_HIP(hipStreamSynchronize(s));
+    if (bad & 1) { set_error("%s: rowptr[0] != 0", who); return SWIFT_ERR_ARG; }
+    if (bad & 2) { set_error("%s: rowptr[m] != nnz (%d)", who, nnz); return SWIFT_ERR_ARG; }
+    if (bad & 4) { set_error("%s: rowptr decreases", who); return SWIFT_ERR_ARG; }
+    if (bad & 8) {
+        set_error("%s: a column index is outside [0, %d)", who, n);
+        return SWIFT_ERR_ARG;
+    }
+    return SWIFT_OK;
+}
+
 // ---------------------------------------------------------- csr -> csc --
 
 __global__ void k_col_hist(const int *__restrict__ colidx, int nnz, int *__restrict__ npc) {
@@ -712,6 +761,9 @@ int build_impl(swift_matrix *M, const int *d_rowptr, const int *d_colidx,
                const void *d_val, hipStream_t s) {
     const int m = M->m, n = M->n, nnz = M->nnz;
     const int vs = vsize_of(M->vtype);
+    // the kernels below scatter by the caller's indices: check them first
+    // (outside the phase times)
+    SWIFT_TRY(check_csr("swift_build", m, n, nnz, d_rowptr, d_colidx, s));
     const double t0 = wall_ms();
     double t_sort = t0, t_block = t0;
     auto stamp = [&]() {
@@ -1125,6 +1177,7 @@ void destroy_matrix(swift_matrix *M) {
 int csr_to_csc_entry(int m, int n, int nnz, swift_dtype vtype, const void *d_val,
                      const int *d_rowptr, const int *d_colidx, void *d_cscval, int *d_cscrow,
                      int *d_cscptr, int *d_npc, hipStream_t s) {
+    SWIFT_TRY(check_csr("swift_csr_to_csc", m, n, nnz, d_rowptr, d_colidx, s));
     CscTemp T;
     SWIFT_TRY(csr_to_csc_impl(m, n, nnz, vtype, d_val, d_rowptr, d_colidx, d_cscval, d_cscrow,
                               d_cscptr, d_npc, nullptr, T, s));

@@ -3,6 +3,7 @@
 // kernel (batched.hip).
 #pragma once
 
+#include "bounds.h"
 #include "swift_internal.h"
 
 #include <type_traits>
@@ -248,12 +249,6 @@ struct RowArgs {
     const unsigned char *rflag;
     int walk_blocks;
 };
-
-// B can be read through a buffer resource with 24-bit row offsets: fewer than
-// 2^24 rows and fewer than 2^31 bytes.
-inline bool buffer_ok(long long nrows, long long ldb, size_t elem) {
-    return nrows < (1LL << 24) && nrows * ldb * (long long)elem < (1LL << 31);
-}
 
 template <class TC>
 __device__ __forceinline__ TC out_scale(const RowArgs &a, int k) {

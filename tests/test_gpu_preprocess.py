@@ -127,3 +127,43 @@ def test_config2_full_build_bit_exact(swift, oracle, cuda):
     M, ref = compare(swift, oracle, torch, d["m"], d["n"], d["rowptr"], d["colidx"], d["val"],
                      torch.float32)
     M.close()
+
+
+@pytest.mark.parametrize("flaw", ["col_high", "col_negative", "rowptr_decreasing",
+                                  "rowptr_end", "rowptr_start"])
+def test_device_csr_is_checked(swift, cuda, flaw):
+    """swift_build and swift_csr_to_csc take device CSR arrays and scatter by
+    their indices: a bad column or row pointer must come back as
+    SWIFT_ERR_ARG from the read-only check pass, before any scatter runs."""
+    import ctypes as C
+    import torch
+    m, n, rp, ci, v = CASES["random_300x200"]
+    rp, ci = rp.copy(), ci.copy()
+    if flaw == "col_high":
+        ci[len(ci) // 2] = n + 1000
+    elif flaw == "col_negative":
+        ci[3] = -1
+    elif flaw == "rowptr_decreasing":
+        rp[10] = rp[12] + 1
+    elif flaw == "rowptr_end":
+        rp[-1] -= 1
+    else:
+        rp[0] = 1
+    print("case", flaw, flush=True)
+    with pytest.raises(swift.SwiftError) as e:
+        swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float32)
+    assert e.value.code == -10
+    dev = torch.device("cuda")
+    d_rp = torch.tensor(rp, device=dev)
+    d_ci = torch.tensor(ci, device=dev)
+    nnz = len(ci)
+    cscptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    cscrow = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    npc = torch.empty(n, dtype=torch.int32, device=dev)
+    P = lambda t: C.c_void_p(t.data_ptr())
+    rc = swift.lib().swift_csr_to_csc(m, n, nnz, swift.F32, None, P(d_rp), P(d_ci), None,
+                                      P(cscrow), P(cscptr), P(npc), swift._stream())
+    assert rc == -10
+    # the same arrays without the flaw still build
+    m, n, rp, ci, v = CASES["random_300x200"]
+    swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float32).close()

@@ -113,19 +113,37 @@ def test_b_sorted_flag(swift, oracle, cuda, layout, dt):
     assert gpu_common.rel_err(C, C64, absC) <= TOL[dt]
 
 
+# Which entries the row view holds decides the launches of a row-view product:
+# powerlaw_3000x2500 (reside part under 1% of its entries, spread over more
+# than 1 row in 64) walks every entry in the row view; tall_2000x64 (12%
+# reside) keeps the reside blocks in a launch of their own after the walk.
+ROWVIEW_HOLDS_ALL = {"powerlaw_3000x2500": True, "tall_2000x64": False}
+
+
+def assert_rowview_path(M, name):
+    inf = M.info()
+    if ROWVIEW_HOLDS_ALL[name]:
+        assert inf.rowview_nnz == inf.nnz and inf.reside_nnz > 0
+    else:
+        assert inf.rowview_nnz == inf.regular_nnz < inf.nnz
+
+
 @pytest.mark.parametrize("kernel", ["DEFAULT", "BALANCED", "ROWVIEW", "FLAT", "WINDOW"])
 @pytest.mark.parametrize("K", [32, 100])
 @pytest.mark.parametrize("layout", ["ROW", "COL"])
-def test_accumulate_flag(swift, oracle, cuda, kernel, K, layout):
+@pytest.mark.parametrize("name", list(ROWVIEW_HOLDS_ALL))
+def test_accumulate_flag(swift, oracle, cuda, kernel, K, layout, name):
     import torch
     if layout == "COL" and kernel in ("ROWVIEW", "FLAT", "WINDOW"):
         pytest.skip("the row-view walks are row-major kernels")
-    m, n, rp, ci, v = CASES["powerlaw_3000x2500"]
+    print("case", name, kernel, K, layout, flush=True)
+    m, n, rp, ci, v = CASES[name]
     vv = np.asarray(v, np.float32)
     B = np.random.default_rng(1).uniform(-1, 1, (n, K)).astype(np.float32)
     C0 = np.random.default_rng(2).uniform(-1, 1, (m, K)).astype(np.float32)
     C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
     M = swift.SwiftMatrix(m, n, rp, ci, vv)
+    assert_rowview_path(M, name)
     col = layout == "COL"
     out = torch.tensor(C0.T.copy() if col else C0, device="cuda")
     M.spmm(torch.tensor(B.T.copy() if col else B, device="cuda"), out=out,
@@ -285,7 +303,7 @@ def test_flat_walk_f64_sorted(swift, oracle, cuda, name, K):
     M.close()
 
 
-@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "tall_2000x64"])
+@pytest.mark.parametrize("name", list(ROWVIEW_HOLDS_ALL))
 def test_output_has_no_stale_values(swift, oracle, cuda, name):
     """C = A*B overwrites every row (the row view stores rows, no zero pass):
     fill C with garbage first."""
@@ -295,6 +313,7 @@ def test_output_has_no_stale_values(swift, oracle, cuda, name):
     B = np.random.default_rng(5).uniform(-1, 1, (n, 128)).astype(np.float32)
     C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
     M = swift.SwiftMatrix(m, n, rp, ci, vv)
+    assert_rowview_path(M, name)
     out = torch.full((m, 128), 1e30, device="cuda")
     M.spmm(torch.tensor(B, device="cuda"), out=out)
     assert gpu_common.rel_err(out.cpu().numpy(), C64, absC) <= 1e-5

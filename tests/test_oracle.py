@@ -266,3 +266,65 @@ def test_block_density_brute_force(oracle, shape):
     if m >= 64 and n >= 64:
         d, dz, _ = oracle.block_density(m, n, rp, ci, 512)
         assert d >= 2 and dz >= 2 * 512
+
+
+def _grad_inputs(gold, K=24, seed=9):
+    m, n = int(gold["m"]), int(gold["n"])
+    rng = np.random.default_rng(seed)
+    dC = rng.standard_normal((m, K))
+    B = rng.standard_normal((n, K))
+    return m, n, dC, B
+
+
+def test_spmm_t_equals_dense_transpose_product(oracle, gold):
+    """dB = A^T dC against a dense numpy A.T @ dC on the golden matrices."""
+    m, n, dC, _ = _grad_inputs(gold)
+    A = matgen.csr_to_dense(m, n, gold["rowptr"], gold["colidx"], gold["val"])
+    got = oracle.spmm_t_f64(m, n, gold["rowptr"], gold["colidx"], gold["val"], dC)
+    np.testing.assert_allclose(got, A.T @ dC, rtol=1e-12, atol=1e-12)
+
+
+def test_sddmm_equals_rowwise_dots(oracle, gold):
+    """dvals[e] = <dC[row_e], B[col_e]> in CSR order, against (dC @ B.T) read
+    at the pattern."""
+    m, n, dC, B = _grad_inputs(gold)
+    rp, ci = gold["rowptr"], gold["colidx"]
+    got = oracle.sddmm_f64(m, n, rp, ci, dC, B)
+    rows = np.repeat(np.arange(m), np.diff(rp))
+    want = (dC @ B.T)[rows, ci]
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_backward_adjoint_identity(oracle):
+    """<dC, A B> = <A^T dC, B> = sum_e v_e dvals_e, the identity the backward
+    rests on, on a rectangular matrix with unsorted rows and duplicate-free
+    random columns."""
+    m, n, rp, ci, v = matgen.random_csr(70, 45, 0.12, 31, shuffle_rows=True)
+    rng = np.random.default_rng(5)
+    K = 7
+    B, dC = rng.standard_normal((n, K)), rng.standard_normal((m, K))
+    C, _ = oracle.spmm_f64(m, n, rp, ci, v, B)
+    dB = oracle.spmm_t_f64(m, n, rp, ci, v, dC)
+    dv = oracle.sddmm_f64(m, n, rp, ci, dC, B)
+    a, b, c = float((dC * C).sum()), float((dB * B).sum()), float(v @ dv)
+    assert abs(a - b) <= 1e-12 * max(1.0, abs(a)) and abs(a - c) <= 1e-12 * max(1.0, abs(a))
+
+
+def test_planted_backward_mistakes_are_caught():
+    """The dense checks above fail for the plausible mistakes: a product with
+    A instead of A^T (on a square matrix, where the shapes still agree), row
+    and column swapped in the SDDMM gather, and one dropped entry."""
+    m, n, rp, ci, v = matgen.random_csr(40, 40, 0.15, 17)
+    rng = np.random.default_rng(3)
+    K = 5
+    dC, B = rng.standard_normal((m, K)), rng.standard_normal((n, K))
+    A = matgen.csr_to_dense(m, n, rp, ci, v)
+    rows = np.repeat(np.arange(m), np.diff(rp))
+    want_dB, want_dv = A.T @ dC, (dC @ B.T)[rows, ci]
+    not_transposed = A @ dC
+    assert not np.allclose(not_transposed, want_dB, rtol=1e-12, atol=1e-12)
+    swapped = np.einsum("ek,ek->e", dC[ci], B[rows])
+    assert not np.allclose(swapped, want_dv, rtol=1e-12, atol=1e-12)
+    dropped = A.copy()
+    dropped[rows[7], ci[7]] -= v[7]
+    assert not np.allclose(dropped.T @ dC, want_dB, rtol=1e-12, atol=1e-12)
