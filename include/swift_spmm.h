@@ -260,6 +260,21 @@ int swift_spmm(const swift_matrix *mat, int K, swift_layout layout,
                swift_dtype dtype, swift_kernel kernel, unsigned flags,
                const void *d_B, void *d_C, swift_stream_t stream);
 
+/* swift_spmm that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  Arguments, results and
+ * errors are those of swift_spmm.  The only product that needs workspace, the
+ * column-major SWIFT_KERNEL_DEFAULT (B and C transposed through it), takes it
+ * from one array of SWIFT_NOALLOC_SCRATCH_BYTES per GPU that the library
+ * holds from its load until the process ends; a product needing more
+ * (round_up(n*K*size(B), 256) + m*K*size(C) + 256 bytes) returns
+ * SWIFT_ERR_ARG.  Every call on one GPU shares that array: calls that overlap
+ * in time on one GPU (other streams or threads) race. */
+#define SWIFT_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int swift_spmm_noalloc(const swift_matrix *mat, int K, swift_layout layout,
+                       swift_dtype dtype, swift_kernel kernel, unsigned flags,
+                       const void *d_B, void *d_C, swift_stream_t stream);
+
 /* swift_spmm with HIP events recorded on `stream` between its launches, the
  * way the reference times only its kernels (src/Swift.h:298-336, where C is
  * zeroed before the timed region): kernel_ms[0] = zeroing of C, [1] = regular

@@ -31,6 +31,9 @@ int spmm_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dty
 int spmm_timed_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dtype,
                     swift_kernel kernel, unsigned flags, const void *B, void *C,
                     const float *kscale, hipStream_t s, float *ms3);
+int spmm_noalloc_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dtype,
+                      swift_kernel kernel, unsigned flags, const void *B, void *C,
+                      hipStream_t s);
 int prepare_values_impl(swift_matrix *M, swift_dtype dtype, hipStream_t s);
 int convert_dense_impl(const float *in, void *out, size_t rows, int K, swift_layout layout,
                        swift_dtype dtype, float *col_scale, hipStream_t s);
@@ -236,6 +239,13 @@ extern "C" int swift_spmm(const swift_matrix *mat, int K, swift_layout layout, s
                           swift_stream_t stream) {
     SWIFT_GUARD(return spmm_impl(mat, K, layout, dtype, kernel, flags, d_B, d_C, nullptr,
                                  S(stream));)
+}
+
+extern "C" int swift_spmm_noalloc(const swift_matrix *mat, int K, swift_layout layout,
+                                  swift_dtype dtype, swift_kernel kernel, unsigned flags,
+                                  const void *d_B, void *d_C, swift_stream_t stream) {
+    SWIFT_GUARD(return spmm_noalloc_impl(mat, K, layout, dtype, kernel, flags, d_B, d_C,
+                                         S(stream));)
 }
 
 extern "C" int swift_spmm_timed(const swift_matrix *mat, int K, swift_layout layout,

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -765,6 +766,52 @@ static int transpose(const T *in, T *out, int rows, int cols, bool accumulate, h
     return SWIFT_OK;
 }
 
+// ------------------------------------------------ allocation-free entry --
+//
+// swift_spmm_noalloc takes every workspace from `scratch`: one array per GPU,
+// allocated with the code object and held until the process ends, so a call
+// allocates nothing (for HIP graph capture and loops that must not allocate).
+// The host reaches it only through the device address hipGetSymbolAddress
+// returns for the current device (looked up once per device and kept):
+// `scratch` itself names a host-side stand-in.  Every call on one GPU shares
+// the array; its transposes overwrite all they later read, so nothing needs
+// clearing, and calls that overlap in time on one GPU race.
+static __device__ __align__(256) unsigned char scratch[SWIFT_NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_addr[MAX_DEVICES];  // device address of scratch, per device
+
+static int scratch_address(void **out) {
+    int dev = 0;
+    SWIFT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        set_error("swift_spmm_noalloc: device %d beyond the %d supported", dev, MAX_DEVICES);
+        return SWIFT_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        SWIFT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("swift_spmm_noalloc: no device address for the scratch array");
+            return SWIFT_ERR_HIP;
+        }
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return SWIFT_OK;
+}
+
+// Caller-provided workspace of the calling thread's current product (set by
+// spmm_noalloc_impl around spmm_impl); nullptr: the matrix's own, grown on
+// demand.
+struct Workspace {
+    void *ptr;
+    size_t bytes;
+};
+static thread_local const Workspace *g_ws = nullptr;
+
 // Column-major C = A*B.  DEFAULT: B is transposed into the matrix's
 // workspace, the row-view product runs row-major, and C is transposed back
 // (3 HBM-bound passes instead of strided scatter atomics).  Other kernels:
@@ -777,9 +824,21 @@ static int spmm_col(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     if (kernel == SWIFT_KERNEL_DEFAULT) {
         const size_t bB = ((size_t)M->n * K * sizeof(TB) + 255) & ~(size_t)255;
         const size_t bC = (size_t)M->m * K * sizeof(TC);
-        SWIFT_TRY(M->scratch.alloc(bB + bC + 256));
-        TB *Bt = M->scratch.as<TB>();
-        TC *Ct = reinterpret_cast<TC *>(static_cast<char *>(M->scratch.ptr) + bB);
+        void *ws;
+        if (g_ws) {
+            if (bB + bC + 256 > g_ws->bytes) {
+                set_error("swift_spmm_noalloc: the column-major product needs %zu bytes of "
+                          "workspace, more than the %zu of the scratch array",
+                          bB + bC + 256, g_ws->bytes);
+                return SWIFT_ERR_ARG;
+            }
+            ws = g_ws->ptr;
+        } else {
+            SWIFT_TRY(M->scratch.alloc(bB + bC + 256));
+            ws = M->scratch.ptr;
+        }
+        TB *Bt = static_cast<TB *>(ws);
+        TC *Ct = reinterpret_cast<TC *>(static_cast<char *>(ws) + bB);
         SWIFT_TRY(transpose<TB>(static_cast<const TB *>(B), Bt, K, M->n, false, s));
         SWIFT_TRY((spmm_row<TV, TB, TC>(M, K, SWIFT_KERNEL_DEFAULT, flags & ~SWIFT_ACCUMULATE, Bt,
                                         Ct, sval, rval, rv_val, kscale, gscale, s)));
@@ -862,6 +921,18 @@ int spmm_timed_impl(const swift_matrix *M, int K, swift_layout layout, swift_dty
         for (int i = 0; i < 3; i++) SWIFT_HIP(hipEventElapsedTime(&ms3[i], pe.ev[i], pe.ev[i + 1]));
     }
     for (auto &e : pe.ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int spmm_noalloc_impl(const swift_matrix *M, int K, swift_layout layout, swift_dtype dtype,
+                      swift_kernel kernel, unsigned flags, const void *B, void *C,
+                      hipStream_t s) {
+    void *p = nullptr;
+    SWIFT_TRY(scratch_address(&p));
+    const Workspace w{p, (size_t)SWIFT_NOALLOC_SCRATCH_BYTES};
+    g_ws = &w;
+    const int rc = spmm_impl(M, K, layout, dtype, kernel, flags, B, C, nullptr, s);
+    g_ws = nullptr;
     return rc;
 }
 

@@ -65,7 +65,7 @@ EXPORTS = (
     "swift_mmio_allinone_ex", "swift_host_free", "swift_generate", "swift_generate_dense", "swift_csr_to_csc",
     "swift_col_sort", "swift_build", "swift_build_host", "swift_destroy",
     "swift_row_idx_sort", "swift_set_segment_items", "swift_get_info", "swift_matrix_stats",
-    "swift_permute_dense", "swift_spmm",
+    "swift_permute_dense", "swift_spmm", "swift_spmm_noalloc",
     "swift_spmm_timed",
     "swift_spmm_host", "swift_prepare_values", "swift_convert_dense", "swift_spmm_scaled",
     "swift_build_transpose", "swift_sddmm", "swift_backward_fused", "swift_update_values",
@@ -107,6 +107,8 @@ def lib():
         L.swift_set_segment_items.argtypes = [_vp, C.c_int, _vp]
         L.swift_spmm.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, _vp, _vp,
                                  _vp]
+        L.swift_spmm_noalloc.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint,
+                                         _vp, _vp, _vp]
         L.swift_spmm_timed.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, _vp,
                                        _vp, _vp, _P(C.c_float)]
         L.swift_spmm_scaled.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_uint, _vp, _vp,
