@@ -163,7 +163,9 @@ struct BwArgs {
     const int *blkstart, *blkstop, *rptr, *rrow, *rsrc;
     const void *rval;
     const int *perm;
-    int K;
+    int K;                     // row length (elements) of G, H and dB
+    int kbase, kcount;         // K slice handled by this launch
+    int dv_add;                // dvals += this slice's partial dots (not the first slice)
     const void *G;             // operand gathered per entry (by column)
     const void *H;             // operand per walked row
     void *dB;                  // walked-row output (fused pass) or nullptr
@@ -263,13 +265,21 @@ __device__ __forceinline__ T multi_group_sum(const T (&d)[N], int sub) {
     return r;
 }
 
+// dvals of one entry: the first K slice stores its partial dot, later slices
+// add theirs (one lane per entry and launch, launches in stream order).
+template <class T>
+__device__ __forceinline__ void put_dval(const BwArgs &a, int src, T r) {
+    T *dv = static_cast<T *>(a.dvals) + src;
+    *dv = a.dv_add ? *dv + r : r;
+}
+
 // N slot rows of one buffer-load step of the backward walk, without branches
 // (only the last slot row can reach past the row's end; its unused slots
 // read zeros through OOB_OFFSET), as in the forward walk's gather_body.
 // DB: acc += v * G[c] for the row's dB.  DV: dvals[src] = <G[c], H[row]>, the
 // slots' partial dots summed over the lane group together (lane sub then
 // holds slot sub / (LPE / UNROLL)).
-template <class T, int VEC, int Q, int LPE, bool DB, bool DV, int UNROLL, int N>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, int UNROLL, int N>
 __device__ __forceinline__ void bw_body(const BwArgs &a, __amdgpu_buffer_rsrc_t rsrc, int base,
                                         int j, int nb, int h, int sub, int mc, int ms, T mv,
                                         const T (&hr)[Q][VEC], T (&acc)[Q][VEC]) {
@@ -288,11 +298,12 @@ __device__ __forceinline__ void bw_body(const BwArgs &a, __amdgpu_buffer_rsrc_t 
         vv[u] = ok ? vs : (T)0;
         const unsigned roff = __umul24((unsigned)c, a.gstride);
 #pragma unroll
-        for (int q = 0; q < Q; q++)
-            load_b_buf<T, VEC, T>(
-                rsrc,
-                ok ? roff + (unsigned)((q * LPE * VEC + sub * VEC) * (int)sizeof(T)) : OOB_OFFSET,
-                g[u][q]);
+        for (int q = 0; q < Q; q++) {
+            const int k0 = q * LPE * VEC + sub * VEC;
+            const bool in = ok && (!MASK || k0 < a.kcount);
+            load_b_buf<T, VEC, T>(rsrc, in ? roff + (unsigned)(k0 * (int)sizeof(T)) : OOB_OFFSET,
+                                  g[u][q]);
+        }
     }
     if (DB) {
 #pragma unroll
@@ -321,31 +332,31 @@ __device__ __forceinline__ void bw_body(const BwArgs &a, __amdgpu_buffer_rsrc_t 
         }
         const T r = multi_group_sum<T, LPE, UNROLL>(d, sub);
         // slots past the row's end (us >= N among them) write nothing
-        if (sub % LEAD == 0 && j + us * EPS + h < nb) static_cast<T *>(a.dvals)[sv] = r;
+        if (sub % LEAD == 0 && j + us * EPS + h < nb) put_dval(a, sv, r);
     }
 }
 
 // The body for nu = min(UNROLL, slot rows left) slot rows (wave-uniform).
-template <class T, int VEC, int Q, int LPE, bool DB, bool DV, int UNROLL, int N>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, int UNROLL, int N>
 __device__ __forceinline__ void bw_pick(int nu, const BwArgs &a, __amdgpu_buffer_rsrc_t rsrc,
                                         int base, int j, int nb, int h, int sub, int mc, int ms,
                                         T mv, const T (&hr)[Q][VEC], T (&acc)[Q][VEC]) {
     if constexpr (N == 1) {
-        bw_body<T, VEC, Q, LPE, DB, DV, UNROLL, 1>(a, rsrc, base, j, nb, h, sub, mc, ms, mv, hr,
-                                                   acc);
+        bw_body<T, VEC, Q, LPE, MASK, DB, DV, UNROLL, 1>(a, rsrc, base, j, nb, h, sub, mc, ms, mv,
+                                                         hr, acc);
     } else {
         if (nu >= N)
-            bw_body<T, VEC, Q, LPE, DB, DV, UNROLL, N>(a, rsrc, base, j, nb, h, sub, mc, ms, mv,
-                                                       hr, acc);
+            bw_body<T, VEC, Q, LPE, MASK, DB, DV, UNROLL, N>(a, rsrc, base, j, nb, h, sub, mc, ms,
+                                                             mv, hr, acc);
         else
-            bw_pick<T, VEC, Q, LPE, DB, DV, UNROLL, N - 1>(nu, a, rsrc, base, j, nb, h, sub, mc,
-                                                           ms, mv, hr, acc);
+            bw_pick<T, VEC, Q, LPE, MASK, DB, DV, UNROLL, N - 1>(nu, a, rsrc, base, j, nb, h, sub,
+                                                                 mc, ms, mv, hr, acc);
     }
 }
 
 // Reside blocks of the walked matrix: column operand G[c] loaded once, each
 // entry gathers H[row]; dvals = <H[row], G[c]>, dB[row] += v * G[c].
-template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV>
 __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
     constexpr int EPS = 64 / LPE;
     const int lane = threadIdx.x & 63;
@@ -356,8 +367,8 @@ __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
     const int start = split ? a.blkstart[blk] : a.rptr[r];
     const int stop = split ? a.blkstop[blk] : a.rptr[r + 1];
     const int c = a.perm[a.n - 1 - r];
-    const T *G = static_cast<const T *>(a.G);
-    const T *H = static_cast<const T *>(a.H);
+    const T *G = static_cast<const T *>(a.G) + a.kbase;
+    const T *H = static_cast<const T *>(a.H) + a.kbase;
     const T *rval = static_cast<const T *>(a.rval);
     // lane-strided layout k = q*LPE*VEC + x*LPE + sub: for a fixed (q, x) the
     // LPE lanes touch LPE consecutive elements, so each dB atomic instruction
@@ -366,7 +377,10 @@ __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
 #pragma unroll
     for (int q = 0; q < Q; q++)
 #pragma unroll
-        for (int x = 0; x < VEC; x++) gc[q][x] = G[(size_t)c * a.K + q * LPE * VEC + x * LPE + sub];
+        for (int x = 0; x < VEC; x++) {
+            const int k = q * LPE * VEC + x * LPE + sub;
+            gc[q][x] = (!MASK || k < a.kcount) ? G[(size_t)c * a.K + k] : (T)0;
+        }
     for (int e0 = start; e0 < stop; e0 += EPS) {
         const int e = e0 + h;
         const bool ok = e < stop;
@@ -376,19 +390,23 @@ __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
 #pragma unroll
             for (int q = 0; q < Q; q++)
 #pragma unroll
-                for (int x = 0; x < VEC; x++)
-                    if (ok) d += H[(size_t)row * a.K + q * LPE * VEC + x * LPE + sub] * gc[q][x];
+                for (int x = 0; x < VEC; x++) {
+                    const int k = q * LPE * VEC + x * LPE + sub;
+                    if (ok && (!MASK || k < a.kcount)) d += H[(size_t)row * a.K + k] * gc[q][x];
+                }
             d = group_sum<T, LPE>(d);
-            if (ok && sub == 0) static_cast<T *>(a.dvals)[a.rsrc[e]] = d;
+            if (ok && sub == 0) put_dval(a, a.rsrc[e], d);
         }
         if (DB && ok) {
             const T v = rval[e];
-            T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K;
+            T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K + a.kbase;
 #pragma unroll
             for (int q = 0; q < Q; q++)
 #pragma unroll
-                for (int x = 0; x < VEC; x++)
-                    atomic_add_bw(drow + q * LPE * VEC + x * LPE + sub, v * gc[q][x]);
+                for (int x = 0; x < VEC; x++) {
+                    const int k = q * LPE * VEC + x * LPE + sub;
+                    if (!MASK || k < a.kcount) atomic_add_bw(drow + k, v * gc[q][x]);
+                }
         }
     }
 }
@@ -399,13 +417,13 @@ __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
 #ifndef SWIFT_BW_UNROLL_LONG
 #define SWIFT_BW_UNROLL_LONG 4
 #endif
-template <class T, int VEC, int Q, int LPE, bool DB, bool DV, bool BUF>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool BUF>
 __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     if constexpr (!DB) {
         if (blockIdx.x >= (unsigned)a.walk_blocks) {
             const int blk =
                 (int)(((blockIdx.x - a.walk_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
-            if (blk < a.nblk) bw_reside_block<T, VEC, Q, LPE, DB, DV>(a, blk);
+            if (blk < a.nblk) bw_reside_block<T, VEC, Q, LPE, MASK, DB, DV>(a, blk);
             return;
         }
     }
@@ -422,8 +440,8 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     int row = st.x, e = st.y;
     const bool carried_in = row < a.m && e > a.rv_ptr[row];
     const T *val = static_cast<const T *>(a.rv_val);
-    const T *G = static_cast<const T *>(a.G);
-    const T *H = static_cast<const T *>(a.H);
+    const T *G = static_cast<const T *>(a.G) + a.kbase;
+    const T *H = static_cast<const T *>(a.H) + a.kbase;
     __amdgpu_buffer_rsrc_t rsrc;
     if constexpr (BUF) rsrc = make_rsrc(G, a.gbytes);
     // staged row ends and entries, as in the forward row-view kernel
@@ -450,7 +468,10 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
         const int row_end = row < r1 ? rend : e1;
         if (DV && e < row_end) {
 #pragma unroll
-            for (int q = 0; q < Q; q++) ldv_stream<T, VEC>(H + (size_t)row * a.K + q * LPE * VEC + sub * VEC, hr[q]);
+            for (int q = 0; q < Q; q++) {
+                const int k0 = q * LPE * VEC + sub * VEC;
+                if (!MASK || k0 < a.kcount) ldv_stream<T, VEC>(H + (size_t)row * a.K + k0, hr[q]);
+            }
         }
         while (e < row_end) {
             if (e - ebase >= 64) {
@@ -471,9 +492,8 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 int src[UNROLL];
                 if constexpr (BUF) {
                     const int nu = min(UNROLL, (nb - j + EPS - 1) / EPS);
-                    bw_pick<T, VEC, Q, LPE, DB, DV, UNROLL, UNROLL>(nu, a, rsrc, (off + j + h) << 2,
-                                                                   j, nb, h, sub, mc, ms, mv, hr,
-                                                                   acc);
+                    bw_pick<T, VEC, Q, LPE, MASK, DB, DV, UNROLL, UNROLL>(
+                        nu, a, rsrc, (off + j + h) << 2, j, nb, h, sub, mc, ms, mv, hr, acc);
                     continue;
                 }
 #pragma unroll
@@ -487,8 +507,9 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                     vv[u] = idx < nb ? vs : (T)0;
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
-                        if (idx < nb)
-                            ldv<T, VEC>(G + (size_t)c * a.K + q * LPE * VEC + sub * VEC, g[u][q]);
+                        const int k0 = q * LPE * VEC + sub * VEC;
+                        if (idx < nb && (!MASK || k0 < a.kcount))
+                            ldv<T, VEC>(G + (size_t)c * a.K + k0, g[u][q]);
                         else
 #pragma unroll
                             for (int v = 0; v < VEC; v++) g[u][q][v] = (T)0;
@@ -521,7 +542,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                         d[u] = x;
                     }
                     const T r = multi_group_sum<T, LPE, UNROLL>(d, sub);
-                    if (sub % LEAD == 0 && j + us * EPS + h < nb) static_cast<T *>(a.dvals)[sv] = r;
+                    if (sub % LEAD == 0 && j + us * EPS + h < nb) put_dval(a, sv, r);
                 }
             }
             e += nb;
@@ -540,12 +561,19 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
 #pragma unroll
                 for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
         if (h == 0) {
-            T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K;
+            T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K + a.kbase;
             const bool split = last || (row == st.x && carried_in);
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const int k0 = q * LPE * VEC + sub * VEC;
-                if (split) {
+                if (MASK) {
+#pragma unroll
+                    for (int v = 0; v < VEC; v++)
+                        if (k0 + v < a.kcount) {
+                            if (split) atomic_add_bw(drow + k0 + v, acc[q][v]);
+                            else drow[k0 + v] = acc[q][v];
+                        }
+                } else if (split) {
 #pragma unroll
                     for (int v = 0; v < VEC; v++) atomic_add_bw(drow + k0 + v, acc[q][v]);
                 } else {
@@ -570,10 +598,10 @@ __global__ void k_zero_rows_bw(const int *__restrict__ rows, int nrows, T *__res
     for (int k = threadIdx.x & 63; k < K; k += 64) d[k] = (T)0;
 }
 
-template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV>
 __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
     const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (blk < a.nblk) bw_reside_block<T, VEC, Q, LPE, DB, DV>(a, blk);
+    if (blk < a.nblk) bw_reside_block<T, VEC, Q, LPE, MASK, DB, DV>(a, blk);
 }
 
 // dvals alone (SDDMM): the reside blocks run as extra blocks of the walk's
@@ -583,49 +611,71 @@ __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
 // atomics, as the forward launch does, cost more than it saved: the
 // transposed 1M-row matrix has many of them, fused backward 1.06 -> 1.13 ms,
 // and the flag took the 8-lane walk from 6 to 5 waves per SIMD.)
-template <class T, int VEC, int Q, int LPE, bool DB, bool DV>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV>
 static int launch_bw(const BwArgs &a0, hipStream_t s) {
     BwArgs a = a0;
     constexpr bool fuse = !DB;
-    if (DB && a.nsplit)
-        hipLaunchKernelGGL(k_zero_rows_bw<T>, dim3(ceil_div((size_t)a.nsplit * 64, 256)), dim3(256),
-                           0, s, a.split_rows, a.nsplit, (T *)a.dB, a.K);
     a.walk_blocks = (int)ceil_div((size_t)a.nseg * 64, 256);
     const size_t res_blocks = ceil_div((size_t)a.nblk * 64, 256);
     const dim3 grid((unsigned)(a.walk_blocks + (fuse ? res_blocks : 0)));
     if (grid.x) {
         if (a.gstride)
-            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, DB, DV, true>), grid, dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, MASK, DB, DV, true>), grid, dim3(256), 0,
+                               s, a);
         else
-            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, DB, DV, false>), grid, dim3(256), 0, s,
-                               a);
+            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, MASK, DB, DV, false>), grid, dim3(256),
+                               0, s, a);
     }
     if (!fuse && res_blocks)
-        hipLaunchKernelGGL((k_bw_reside<T, VEC, Q, LPE, DB, DV>), dim3(res_blocks), dim3(256), 0,
-                           s, a);
+        hipLaunchKernelGGL((k_bw_reside<T, VEC, Q, LPE, MASK, DB, DV>), dim3(res_blocks),
+                           dim3(256), 0, s, a);
     SWIFT_CHECK_LAUNCH();
     return SWIFT_OK;
 }
 
+// Any K: the slices of plan_row_slices (the forward walk's planner), each one
+// launch with its own lane shape; the full-vector shapes need K % VEC == 0
+// and 16-byte aligned operands, the rest of K takes masked scalar slices of
+// up to 64.  dB rows split by a segment boundary are zeroed once over all of
+// K; every slice stores its own columns of dB, and dvals takes the first
+// slice's partial dots, then each later slice's added on.
 template <class T, bool DB, bool DV>
-static int dispatch_bw(const BwArgs &a, hipStream_t s) {
+static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
     constexpr int V = V16<T>::N;
-    const int K = a.K;
-    if (K % V != 0) {
-        set_error("backward kernels need K %% %d == 0 (K=%d)", V, K);
+    const int K = a0.K;
+    BwArgs a = a0;
+    const bool aligned = K % V == 0 &&
+                         (((uintptr_t)a.G | (uintptr_t)a.H | (uintptr_t)a.dB) % 16) == 0;
+    if (DB && a.nsplit)
+        hipLaunchKernelGGL(k_zero_rows_bw<T>, dim3(ceil_div((size_t)a.nsplit * 64, 256)), dim3(256),
+                           0, s, a.split_rows, a.nsplit, (T *)a.dB, K);
+    const auto slices = plan_row_slices(K, V, aligned);
+    for (size_t i = 0; i < slices.size(); i++) {
+        const RowTarget &t = slices[i].second;
+        a.kbase = slices[i].first;
+        a.kcount = t.mask ? std::min(64, K - a.kbase) : t.vec * t.lpe * t.q;
+        a.dv_add = i > 0 ? 1 : 0;
+        if (a.gstride) a.gbytes = (unsigned)((a0.gbytes / sizeof(T) - (size_t)a.kbase) * sizeof(T));
+        if (t.mask) {
+            SWIFT_TRY((launch_bw<T, 1, 1, 64, true, DB, DV>(a, s)));
+            continue;
+        }
+#define SWIFT_BW_CASE(Q_, L_)                                                 \
+        if (t.q == Q_ && t.lpe == L_) {                                      \
+            SWIFT_TRY((launch_bw<T, V, Q_, L_, false, DB, DV>(a, s)));       \
+            continue;                                                        \
+        }
+        SWIFT_BW_CASE(4, 64)
+        SWIFT_BW_CASE(2, 64)
+        SWIFT_BW_CASE(1, 64)
+        SWIFT_BW_CASE(1, 32)
+        SWIFT_BW_CASE(1, 16)
+        SWIFT_BW_CASE(1, 8)
+#undef SWIFT_BW_CASE
+        set_error("no backward kernel for vec=%d q=%d lpe=%d", t.vec, t.q, t.lpe);
         return SWIFT_ERR_UNSUPPORTED;
     }
-    switch (K / V) {
-        case 8: return launch_bw<T, V, 1, 8, DB, DV>(a, s);
-        case 16: return launch_bw<T, V, 1, 16, DB, DV>(a, s);
-        case 32: return launch_bw<T, V, 1, 32, DB, DV>(a, s);
-        case 64: return launch_bw<T, V, 1, 64, DB, DV>(a, s);
-        case 128: return launch_bw<T, V, 2, 64, DB, DV>(a, s);
-        case 256: return launch_bw<T, V, 4, 64, DB, DV>(a, s);
-        default: break;
-    }
-    set_error("backward kernels support K/%d in {8,16,32,64,128,256} (K=%d)", V, K);
-    return SWIFT_ERR_UNSUPPORTED;
+    return SWIFT_OK;
 }
 
 static BwArgs walk_args(const swift_matrix *W, int K) {

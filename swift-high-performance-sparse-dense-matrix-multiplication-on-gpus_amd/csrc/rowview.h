@@ -7,6 +7,8 @@
 #include "swift_internal.h"
 
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 namespace swift {
 
@@ -249,6 +251,50 @@ struct RowArgs {
     const unsigned char *rflag;
     int walk_blocks;
 };
+
+struct RowTarget {
+    int vec, q, lpe;
+    bool mask;
+};
+
+// Splits [0, K) into slices handled by one kernel each: the largest
+// 16-byte-vector shape that fits, then masked scalar slices of <= 64.  Shared
+// by the forward and backward walks.
+inline std::vector<std::pair<int, RowTarget>> plan_row_slices(int K, int vec, bool aligned) {
+    std::vector<std::pair<int, RowTarget>> out;
+    int k = 0;
+    if (aligned) {
+        static const int qs[] = {4, 2, 1};
+        while (K - k >= vec * 4) {
+            int rem = K - k;
+            bool done = false;
+            for (int q : qs) {
+                if (rem >= vec * 64 * q) {
+                    out.push_back({k, {vec, q, 64, false}});
+                    k += vec * 64 * q;
+                    done = true;
+                    break;
+                }
+            }
+            if (done) continue;
+            for (int lpe = 32; lpe >= 4; lpe >>= 1) {
+                if (lpe == 4 && vec != 8) break;
+                if (rem >= vec * lpe) {
+                    out.push_back({k, {vec, 1, lpe, false}});
+                    k += vec * lpe;
+                    done = true;
+                    break;
+                }
+            }
+            if (!done) break;
+        }
+    }
+    while (k < K) {
+        out.push_back({k, {1, 1, 64, true}});
+        k += std::min(64, K - k);
+    }
+    return out;
+}
 
 template <class TC>
 __device__ __forceinline__ TC out_scale(const RowArgs &a, int k) {

@@ -431,49 +431,6 @@ static inline void mark(int i, hipStream_t s) {
 }
 
 
-struct RowTarget {
-    int vec, q, lpe;
-    bool mask;
-};
-
-// Splits [0, K) into slices handled by one kernel each: the largest
-// 16-byte-vector shape that fits, then masked scalar slices of <= 64.
-static std::vector<std::pair<int, RowTarget>> plan_row_slices(int K, int vec, bool aligned) {
-    std::vector<std::pair<int, RowTarget>> out;
-    int k = 0;
-    if (aligned) {
-        static const int qs[] = {4, 2, 1};
-        while (K - k >= vec * 4) {
-            int rem = K - k;
-            bool done = false;
-            for (int q : qs) {
-                if (rem >= vec * 64 * q) {
-                    out.push_back({k, {vec, q, 64, false}});
-                    k += vec * 64 * q;
-                    done = true;
-                    break;
-                }
-            }
-            if (done) continue;
-            for (int lpe = 32; lpe >= 4; lpe >>= 1) {
-                if (lpe == 4 && vec != 8) break;
-                if (rem >= vec * lpe) {
-                    out.push_back({k, {vec, 1, lpe, false}});
-                    k += vec * lpe;
-                    done = true;
-                    break;
-                }
-            }
-            if (!done) break;
-        }
-    }
-    while (k < K) {
-        out.push_back({k, {1, 1, 64, true}});
-        k += std::min(64, K - k);
-    }
-    return out;
-}
-
 enum RowKind { ROW_TILES = 0, ROW_RESIDE = 1, ROW_VIEW = 2, ROW_FLAT = 3, ROW_WIN = 4 };
 
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>

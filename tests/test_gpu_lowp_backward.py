@@ -155,13 +155,22 @@ def test_f16_scaled_storage(swift, oracle, cuda, name, K, layout):
     M.close()
 
 
-@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "equalrows_100x70",
-                                  "unsorted_rows_64x96", "narrow_50x20"])
-@pytest.mark.parametrize("K", [32, 128])
-@pytest.mark.parametrize("dt", ["f32", "f64"])
-def test_backward(swift, oracle, cuda, name, K, dt):
+def _operand(torch, a, offset):
+    """a on the GPU; offset > 0 places it `offset` elements into a larger
+    buffer, so its address is not 16-byte aligned."""
+    if not offset:
+        return torch.tensor(a, device="cuda")
+    flat = torch.zeros(a.size + offset, dtype=torch.float32 if a.dtype == np.float32
+                       else torch.float64, device="cuda")
+    flat[offset:] = torch.tensor(a.ravel(), device="cuda")
+    return flat[offset:].view(a.shape)
+
+
+def check_backward(swift, oracle, name, K, dt, offset=0):
+    """Fused backward, the transposed-layout product and SDDMM against the
+    FP64 oracle: dB = A^T dC within tol of (|A|^T|dC|), dvals within tol of
+    <|dC[row]|, |B[col]|>."""
     import torch
-    print("case", name, K, dt, flush=True)
     m, n, rp, ci, v = CASES[name]
     npdt = np.float32 if dt == "f32" else np.float64
     dtype = torch.float32 if dt == "f32" else torch.float64
@@ -177,8 +186,8 @@ def test_backward(swift, oracle, cuda, name, K, dt):
     absv = oracle.sddmm_f64(m, n, rp, ci, np.abs(dC.astype(np.float64)),
                             np.abs(B.astype(np.float64)))
     M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=dtype)
-    dCt = torch.tensor(dC, device="cuda")
-    Bt = torch.tensor(B, device="cuda")
+    dCt = _operand(torch, dC, offset)
+    Bt = _operand(torch, B, offset)
     dB_f, dv_f = M.backward_fused(dCt, Bt)
     dB_t = M.transpose().spmm(dCt)
     dv_s = M.sddmm(dCt, Bt)
@@ -188,6 +197,34 @@ def test_backward(swift, oracle, cuda, name, K, dt):
         got = dv.cpu().numpy().astype(np.float64)
         assert float(np.max(np.abs(got - dv64) / (absv + 1e-30), initial=0)) <= tol
     M.close()
+
+
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "equalrows_100x70",
+                                  "unsorted_rows_64x96", "narrow_50x20"])
+@pytest.mark.parametrize("K", [32, 128])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_backward(swift, oracle, cuda, name, K, dt):
+    print("case", name, K, dt, flush=True)
+    check_backward(swift, oracle, name, K, dt)
+
+
+# the K sweep of test/irregular_K (script/Run_irregular_K.sh) plus K = 100 and
+# small K; each K runs as slices of one lane shape per launch, the part of K
+# past the last full vector shape in masked scalar slices
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "tall_2000x64"])
+@pytest.mark.parametrize("K", [1, 3, 24, 48, 96, 100, 192, 384, 768])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_backward_irregular_k(swift, oracle, cuda, name, K, dt):
+    print("case", name, K, dt, flush=True)
+    check_backward(swift, oracle, name, K, dt)
+
+
+@pytest.mark.parametrize("K", [32, 100])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_backward_unaligned_operands(swift, oracle, cuda, K, dt):
+    """dC and B one element off a 16-byte boundary: every slice is masked."""
+    print("case unaligned", K, dt, flush=True)
+    check_backward(swift, oracle, "powerlaw_3000x2500", K, dt, offset=1)
 
 
 def test_transpose_layout_bit_exact(swift, oracle, cuda):
@@ -225,13 +262,14 @@ def test_update_values(swift, oracle, cuda):
 
 
 @pytest.mark.parametrize("fused", [True, False])
-def test_autograd_finite_differences(swift, cuda, fused):
+@pytest.mark.parametrize("K", [16, 24, 100])
+def test_autograd_finite_differences(swift, cuda, fused, K):
     """d loss / d vals and d loss / d B of loss = <W, A(vals) B> against FP64
     central differences on a small power-law graph (induced subgraph size)."""
     import torch
     d = swift.generate(4, scale=0.01, seed=1)  # 2,000 nodes
     m, n, rp, ci, v = d["m"], d["n"], d["rowptr"], d["colidx"], d["val"]
-    K = 16
+    print("case fd", K, fused, flush=True)
     M = swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float64)
     Fn = swift.spmm_function()
     rng = np.random.default_rng(0)
