@@ -101,6 +101,14 @@ typedef struct {
     /* Entries in the row view that the row-view kernels walk (ours):
      * regular_nnz, or nnz when the reside part is walked with them. */
     int rowview_nnz;
+    /* Schedule of SWIFT_KERNEL_DEFAULT (ours): segment_items = items per
+     * merge-path segment set by swift_set_segment_items or the autotuner (0:
+     * the default tables); for products of tuned_K columns (0: none tuned)
+     * the autotuner's walk (SWIFT_KERNEL_DEFAULT = row-view walk,
+     * SWIFT_KERNEL_WINDOW = window walk) and slot rows per step (0: the
+     * rule). */
+    int segment_items, tuned_K, tuned_slots;
+    swift_kernel tuned_kernel;
 } swift_info;
 
 /* ------------------------------------------------------------------ host */
@@ -348,11 +356,13 @@ typedef struct swift_batch swift_batch; /* opaque */
  * persistent launch.  No counterpart in the reference, which runs one
  * matrix per process (test/k-32/FP32/main.cu:32-72) with two launches per
  * product (test/k-32/FP32/Swift_gpu_row_major.h:264,280); here one work list
- * of (matrix, segment) items serves every matrix.  The autotuner keys matrices by
- * (K, log2 mean nonzeros per row, reside share), times the largest of each
- * key over segment sizes {64, 128, 256, 512} and applies the fastest to all
- * of that key (cached per process).  Operand pointers and 16-byte alignment
- * are checked at launch. */
+ * of (matrix, segment) items serves every matrix.  The autotuner keys
+ * matrices by (K, log2 mean nonzeros per row, reside share, whether the row
+ * view holds the reside part), times the largest of each key over segment
+ * sizes {64, 128, 256, 512} x two slot-row counts per step (the rule and 8
+ * for K <= 64, 4 above) and applies the fastest to all of that key (cached
+ * per process).  Operand pointers and 16-byte alignment are checked at
+ * launch. */
 int swift_batch_create(swift_batch **out, swift_matrix *const *mats,
                        const int *Ks, int count, swift_stream_t stream);
 /* All C_i = A_i * B_i (row-major, original column order, F32) in one
@@ -368,10 +378,14 @@ int swift_batch_params(const swift_batch *batch, int *work_units,
                        int *depth, int *block_threads);
 void swift_batch_destroy(swift_batch *batch);
 
-/* Time the row-view product of one matrix for segment sizes {64, 128, 256,
- * 512} on the caller's operands (C is overwritten) and keep the fastest.
- * Has no counterpart in the reference, whose tiling constants are fixed at
- * compile time (test/k-32/FP32/main.cu:21-23). */
+/* Time the SWIFT_KERNEL_DEFAULT product of one matrix on the caller's
+ * operands (C is overwritten) for every candidate schedule and keep the
+ * fastest for products of this K and dtype: segment sizes {64, 128, 256,
+ * 512} x the slot rows per step the product's lane shape can take (4, 8,
+ * 16), and the window walk (row-major).  best_items receives the segment
+ * size; swift_info reports the rest.  Has no counterpart in the reference,
+ * whose tiling constants are fixed at compile time
+ * (test/k-32/FP32/main.cu:21-23). */
 int swift_autotune(swift_matrix *mat, int K, swift_dtype dtype,
                    swift_layout layout, const void *d_B, void *d_C,
                    swift_stream_t stream, int *best_items);

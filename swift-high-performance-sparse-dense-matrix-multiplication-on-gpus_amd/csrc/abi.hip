@@ -221,6 +221,10 @@ extern "C" int swift_get_info(const swift_matrix *M, swift_info *o) {
     o->sortsrc = M->sortsrc; o->reside_src = M->reside_src;
     for (int i = 0; i < 4; i++) o->build_ms[i] = M->build_ms[i];
     o->rowview_nnz = M->rv_nnz;
+    o->segment_items = M->user_items;
+    o->tuned_K = M->tuned_K;
+    o->tuned_slots = M->tuned_slots;
+    o->tuned_kernel = M->tuned_kernel;
     return SWIFT_OK;
 }
 

@@ -62,10 +62,15 @@ __global__ __launch_bounds__(256) void k_batch_zero(const RowArgs *__restrict__ 
     for (int k = threadIdx.x & 63; k < K; k += 64) c[k] = 0.0f;
 }
 
-template <int LPE, bool BUF>
+template <int LPE, bool BUF, int SLOTS = 0>
 __device__ __forceinline__ void batch_walk(const RowArgs &a, int sid) {
-    rowview_walk<float, float, float, 4, 1, LPE, false, BUF>(a, sid);
+    rowview_walk<float, float, float, 4, 1, LPE, false, BUF, false, false, SLOTS>(a, sid);
 }
+
+// Slot-row count the batched walk can run besides the rule for a K: 8 for
+// K = 32 and 64 (rule 4), 4 for K = 128 and 256 (rule 8).  Larger counts
+// would raise the register count of the one kernel that holds every shape.
+constexpr int batch_alt_slots(int K) { return K <= 64 ? 8 : 4; }
 
 // Next CLAIM items of a wave: items range[x] .. range[x+1]-1 belong to XCD x
 // and counter[x * CPAD] counts the ones handed out.  A wave drains its own
@@ -106,10 +111,15 @@ __global__ __launch_bounds__(256) void k_batch_walk(const RowArgs *__restrict__ 
             const int mi = __builtin_amdgcn_readfirstlane(w.x);
             const int sid = __builtin_amdgcn_readfirstlane(w.y);
             const RowArgs &a = desc[mi];
-            switch (a.ldb) {
+            // lane shape by K, slot rows by the autotuner (0: the rule)
+            switch (a.ldb | (a.slots << 12)) {
                 case 32: batch_walk<8, BUF>(a, sid); break;
+                case 32 | (batch_alt_slots(32) << 12): batch_walk<8, BUF, batch_alt_slots(32)>(a, sid); break;
                 case 64: batch_walk<16, BUF>(a, sid); break;
+                case 64 | (batch_alt_slots(64) << 12): batch_walk<16, BUF, batch_alt_slots(64)>(a, sid); break;
                 case 128: batch_walk<32, BUF>(a, sid); break;
+                case 128 | (batch_alt_slots(128) << 12): batch_walk<32, BUF, batch_alt_slots(128)>(a, sid); break;
+                case 256 | (batch_alt_slots(256) << 12): batch_walk<64, BUF, batch_alt_slots(256)>(a, sid); break;
                 default: batch_walk<64, BUF>(a, sid); break;  // 256
             }
         }
@@ -138,7 +148,11 @@ __global__ __launch_bounds__(256) void k_batch_reside(const RowArgs *__restrict_
 // reside entries): matrices of one key walk the same entry sets with the
 // same lane shape.
 using TuneKey = std::tuple<int, int, int, int>;
-static std::map<TuneKey, int> g_tuned;
+struct TuneChoice {
+    int W = 128, slots = 0;
+    swift_kernel kernel = SWIFT_KERNEL_DEFAULT;
+};
+static std::map<TuneKey, TuneChoice> g_tuned;
 static std::mutex g_tune_mu;
 
 static TuneKey tune_key(const swift_matrix *M, int K) {
@@ -149,20 +163,50 @@ static TuneKey tune_key(const swift_matrix *M, int K) {
     return TuneKey(K, dens, res, M->rv_all ? 1 : 0);
 }
 
-// Times the row-view product of M for each candidate segment size on the
-// given operands and keeps the fastest (median of 3 after one warm-up).
+// Makes c the matrix's schedule for products of K columns with elem-byte B
+// elements: its segment table, slot rows per step and walk.
+static int apply_choice(swift_matrix *M, int K, size_t elem, const TuneChoice &c, hipStream_t s) {
+    if (M->mp_items != c.W || !M->user_items) SWIFT_TRY(build_segments(M, c.W, s));
+    M->user_items = c.W;
+    M->tuned_K = K;
+    M->tuned_elem = (int)elem;
+    M->tuned_slots = c.slots;
+    M->tuned_kernel = c.kernel;
+    return SWIFT_OK;
+}
+
+// Times the SWIFT_KERNEL_DEFAULT product of M on the given operands for every
+// candidate schedule and keeps the fastest (median of 3 after one warm-up).
+// Candidates: segment sizes {64, 128, 256, 512} x the slot-row counts of the
+// product's lane shape (slot_candidates), plus the window walk (LDS-staged
+// column windows) at each size.  for_batch: only what the batched walk runs,
+// the rule and batch_alt_slots, with the row-view walk.
 int autotune_impl(swift_matrix *M, int K, swift_dtype dtype, swift_layout layout, const void *B,
-                  void *C, hipStream_t s, int *best_out) {
-    static const int cands[] = {64, 128, 256, 512};
+                  void *C, hipStream_t s, bool for_batch, TuneChoice *best_out) {
+    static const int sizes[] = {64, 128, 256, 512};
+    const size_t elem = value_size(dtype);
+    std::vector<TuneChoice> cands;
+    std::vector<int> slots;
+    if (for_batch) {
+        slots = {0, batch_alt_slots(K)};
+    } else {
+        const int vec = vec_of(elem);
+        const bool aligned = K % vec == 0 && (((uintptr_t)B | (uintptr_t)C) % 16) == 0;
+        slots = slot_candidates(K, elem, aligned);
+    }
+    for (int W : sizes) {
+        for (int n : slots) cands.push_back({W, n, SWIFT_KERNEL_DEFAULT});
+        if (!for_batch && layout == SWIFT_ROW_MAJOR)
+            cands.push_back({W, 0, SWIFT_KERNEL_WINDOW});
+    }
     hipEvent_t e0, e1;
     SWIFT_HIP(hipEventCreate(&e0));
     SWIFT_HIP(hipEventCreate(&e1));
-    int best = M->mp_items ? M->mp_items : 128;
+    TuneChoice best;
     float best_ms = 1e30f;
     int rc = SWIFT_OK;
-    for (int W : cands) {
-        if ((rc = build_segments(M, W, s)) != SWIFT_OK) break;
-        M->user_items = W;  // time this table, not the narrow-row one
+    for (const TuneChoice &c : cands) {
+        if ((rc = apply_choice(M, K, elem, c, s)) != SWIFT_OK) break;
         std::vector<float> t;
         for (int r = 0; r < 4 && rc == SWIFT_OK; r++) {
             (void)hipEventRecord(e0, s);
@@ -177,14 +221,13 @@ int autotune_impl(swift_matrix *M, int K, swift_dtype dtype, swift_layout layout
         std::sort(t.begin(), t.end());
         if (t[1] < best_ms) {
             best_ms = t[1];
-            best = W;
+            best = c;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != SWIFT_OK) return rc;
-    SWIFT_TRY(build_segments(M, best, s));
-    M->user_items = best;
+    SWIFT_TRY(apply_choice(M, K, elem, best, s));
     if (best_out) *best_out = best;
     return SWIFT_OK;
 }
@@ -195,7 +238,7 @@ using namespace swift;
 
 struct swift_batch {
     std::vector<swift_matrix *> mats;
-    std::vector<int> Ks, W;
+    std::vector<int> Ks, W, slots;  // per matrix: K, segment size, slot rows (0: rule)
     std::vector<RowArgs> desc_h;
     RowArgs *desc = nullptr;
     int2 *items = nullptr, *res_items = nullptr, *split = nullptr;
@@ -272,6 +315,7 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
         b->mats.assign(mats, mats + count);
         b->Ks.assign(Ks, Ks + count);
         b->W.assign(count, 128);
+        b->slots.assign(count, 0);
         // autotune one representative per key
         std::map<TuneKey, int> rep;
         for (int i = 0; i < count; i++) {
@@ -280,26 +324,35 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
             if (it == rep.end() || mats[i]->nnz > mats[it->second]->nnz) rep[k] = i;
         }
         for (auto &kv : rep) {
-            int W = 0;
+            TuneChoice c;
+            bool have = false;
             {
                 std::lock_guard<std::mutex> g(g_tune_mu);
                 auto it = g_tuned.find(kv.first);
-                if (it != g_tuned.end()) W = it->second;
+                if (it != g_tuned.end()) {
+                    c = it->second;
+                    have = true;
+                }
             }
-            if (!W) {
+            if (!have) {
                 swift_matrix *M = mats[kv.second];
                 const int K = Ks[kv.second];
                 Buffer Bt, Ct;
                 if ((rc = Bt.alloc(sizeof(float) * (size_t)M->n * K + 16)) != SWIFT_OK) break;
                 if ((rc = Ct.alloc(sizeof(float) * (size_t)M->m * K + 16)) != SWIFT_OK) break;
                 if ((rc = fill_bytes_zero(Bt.ptr, sizeof(float) * (size_t)M->n * K, s)) != SWIFT_OK) break;
-                if ((rc = autotune_impl(M, K, SWIFT_F32, SWIFT_ROW_MAJOR, Bt.ptr, Ct.ptr, s, &W)) != SWIFT_OK) break;
+                if ((rc = autotune_impl(M, K, SWIFT_F32, SWIFT_ROW_MAJOR, Bt.ptr, Ct.ptr, s, true,
+                                        &c)) != SWIFT_OK)
+                    break;
                 (void)hipStreamSynchronize(s);
                 std::lock_guard<std::mutex> g(g_tune_mu);
-                g_tuned[kv.first] = W;
+                g_tuned[kv.first] = c;
             }
             for (int i = 0; i < count; i++)
-                if (tune_key(mats[i], Ks[i]) == kv.first) b->W[i] = W;
+                if (tune_key(mats[i], Ks[i]) == kv.first) {
+                    b->W[i] = c.W;
+                    b->slots[i] = c.slots;
+                }
         }
         if (rc == SWIFT_OK) {
             // a matrix listed more than once keeps one segment table
@@ -309,8 +362,8 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
                 b->W[i] = it->second;
             }
             for (int i = 0; i < count && rc == SWIFT_OK; i++)
-                if (mats[i]->mp_items != b->W[i]) rc = build_segments(mats[i], b->W[i], s);
-            for (int i = 0; i < count && rc == SWIFT_OK; i++) mats[i]->user_items = b->W[i];
+                rc = apply_choice(mats[i], Ks[i], sizeof(float),
+                                  TuneChoice{b->W[i], b->slots[i], SWIFT_KERNEL_DEFAULT}, s);
         }
         if (rc == SWIFT_OK) {
             // flattened work lists (host) -> device
@@ -337,6 +390,7 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
                 }
                 for (int r : rows) split.push_back(make_int2(i, r));
                 b->desc_h.push_back(row_args_of(M, b->Ks[i]));
+                b->desc_h.back().slots = b->slots[i];
                 if (!b->desc_h.back().bstride) b->buf = false;
             }
             auto up = [&](void **dst, const void *src, size_t bytes) -> int {
@@ -523,8 +577,11 @@ extern "C" int swift_autotune(swift_matrix *mat, int K, swift_dtype dtype, swift
         return SWIFT_ERR_ARG;
     }
     try {
-        return autotune_impl(mat, K, dtype, layout, d_B, d_C, reinterpret_cast<hipStream_t>(stream),
-                             best_items);
+        TuneChoice c;
+        const int rc = autotune_impl(mat, K, dtype, layout, d_B, d_C,
+                                     reinterpret_cast<hipStream_t>(stream), false, &c);
+        if (rc == SWIFT_OK && best_items) *best_items = c.W;
+        return rc;
     } catch (...) {
         set_error("swift_autotune: host allocation failed");
         return SWIFT_ERR_ALLOC;

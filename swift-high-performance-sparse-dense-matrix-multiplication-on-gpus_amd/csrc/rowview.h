@@ -250,6 +250,9 @@ struct RowArgs {
     // walk_blocks on run the reside blocks
     const unsigned char *rflag;
     int walk_blocks;
+    // row-view walk: slot rows per step chosen by the autotuner (0: the rule
+    // of rule_slots)
+    int slots;
 };
 
 struct RowTarget {
@@ -359,9 +362,10 @@ __device__ __forceinline__ void lane_scales(const RowArgs &a, int sub, TC (&osc)
 }
 
 // Writes one finished row: plain (or read-modify-write when accumulating)
-// vector stores, or atomics for a row split across segments.  osc: the
-// lane's output scales when hoisted (lane_scales), else read per store.
-template <class TC, int VEC, int Q, int LPE, bool MASK>
+// vector stores, or atomics for a row split across segments.  SCALE (F16
+// storage only): the output is scaled per k, by the lane's hoisted scales osc
+// (lane_scales) or, without them, read per store.
+template <class TC, int VEC, int Q, int LPE, bool MASK, bool SCALE>
 __device__ __forceinline__ void put_row(const RowArgs &a, int row, int sub, bool split,
                                         TC (&acc)[Q][VEC], const TC (*osc)[VEC] = nullptr) {
     TC *crow = static_cast<TC *>(a.C) + (size_t)row * a.ldc + a.kbase;
@@ -370,8 +374,10 @@ __device__ __forceinline__ void put_row(const RowArgs &a, int row, int sub, bool
         const int k0 = q * LPE * VEC + sub * VEC;
         TC x[VEC];
 #pragma unroll
-        for (int v = 0; v < VEC; v++)
-            x[v] = acc[q][v] * (osc ? osc[q][v] : out_scale<TC>(a, k0 + v));
+        for (int v = 0; v < VEC; v++) {
+            if constexpr (SCALE) x[v] = acc[q][v] * (osc ? osc[q][v] : out_scale<TC>(a, k0 + v));
+            else x[v] = acc[q][v];
+        }
         if (MASK) {
 #pragma unroll
             for (int v = 0; v < VEC; v++)
@@ -560,15 +566,47 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 //
 // FLAGS: a.rflag marks rows that reside blocks of the same launch also add
 // into; the walk stores them with atomics.
+//
+// SLOTS: slot rows (B-row loads in flight per lane group) per step; 0 takes
+// rule_slots, other values come from the autotuner.
+constexpr int rule_slots(int lpe, int q, int elem, bool win = false) {
+    return (lpe >= 32 && q <= 2 && !win) ? SWIFT_UNROLL_LONG
+           : elem == 2                   ? SWIFT_UNROLL_NARROW
+                                         : SWIFT_UNROLL_WIDE;
+}
+
+// Slot-row counts the autotuner may choose for a lane shape: one chunk per
+// lane (Q == 1) and at most one staged batch of 64 entries per step.
+constexpr bool slots_ok(int lpe, int q, int slots) {
+    return q == 1 && (slots == 4 || slots == 8 || slots == 16) && (64 / lpe) * slots <= 64;
+}
+
+// Elements per lane load for B elements of elem bytes (Vec16 on the host).
+inline int vec_of(size_t elem) { return elem == 8 ? 2 : elem == 4 ? 4 : SWIFT_NARROW_VEC; }
+
+// Slot-row counts worth timing for a product of K columns with elem-byte B
+// elements: 0 (the rule) and every other count the first full-vector slice's
+// lane shape can take.
+inline std::vector<int> slot_candidates(int K, size_t elem, bool aligned) {
+    std::vector<int> out{0};
+    for (auto &sl : plan_row_slices(K, vec_of(elem), aligned)) {
+        const RowTarget &t = sl.second;
+        if (t.mask) continue;
+        for (int n : {4, 8, 16})
+            if (slots_ok(t.lpe, t.q, n) && n != rule_slots(t.lpe, t.q, (int)elem)) out.push_back(n);
+        break;
+    }
+    return out;
+}
+
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF,
-          bool WIN = false, bool FLAGS = false>
+          bool WIN = false, bool FLAGS = false, int SLOTS = 0>
 __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window win = Window{}) {
     static_assert(!WIN || (BUF && !MASK), "the window walk is a buffer-load, unmasked walk");
     constexpr int EPS = 64 / LPE;
     // B-row loads in flight per lane group and step
-    constexpr int UNROLL = (LPE >= 32 && Q <= 2 && !WIN) ? SWIFT_UNROLL_LONG
-                           : sizeof(TB) == 2             ? SWIFT_UNROLL_NARROW
-                                                         : SWIFT_UNROLL_WIDE;
+    constexpr int UNROLL = SLOTS ? SLOTS : rule_slots(LPE, Q, (int)sizeof(TB), WIN);
+    static_assert(EPS * UNROLL <= 64, "a step takes at most one staged batch of entries");
     constexpr int STEP = EPS * UNROLL;  // entries per iteration
     // BF16 A and B: two entries' products per v_dot2_f32_bf16 (gather_body)
     constexpr bool PAIR = std::is_same<TV, __hip_bfloat16>::value &&
@@ -740,7 +778,8 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
         // rows that other waves (or reside blocks) also add into take atomics
         bool shared = last || (row == st.x && carried_in);
         if constexpr (FLAGS) shared = shared || __builtin_amdgcn_readlane(rfl_l, row - rbase);
-        if (h == 0) put_row<TC, VEC, Q, LPE, MASK>(a, row, sub, shared, acc, HOIST ? osc : nullptr);
+        if (h == 0)
+            put_row<TC, VEC, Q, LPE, MASK, HOIST>(a, row, sub, shared, acc, HOIST ? osc : nullptr);
         else
 #pragma unroll
             for (int q = 0; q < Q; q++)
@@ -766,6 +805,7 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, int U>
 __device__ __forceinline__ void flat_walk(const RowArgs &a, int w) {
     static_assert(U <= LPE, "a chunk must fit in one staged batch");
+    constexpr bool SCALE = std::is_same<TB, __half>::value;  // F16 storage scales its output
     constexpr int SPAN = (LPE / U) * U;  // staged entries consumed per batch
     const int lane = threadIdx.x & 63;
     const int h = lane / LPE, sub = lane % LPE;
@@ -837,7 +877,8 @@ __device__ __forceinline__ void flat_walk(const RowArgs &a, int w) {
         for (int u = 0; u < U; u++) {
             if (rel + u < n) {
                 if (rr[u] != cur) {
-                    put_row<TC, VEC, Q, LPE, MASK>(a, cur, sub, cur == first && carried, acc);
+                    put_row<TC, VEC, Q, LPE, MASK, SCALE>(a, cur, sub, cur == first && carried,
+                                                          acc);
                     cur = rr[u];
                 }
 #pragma unroll
@@ -847,7 +888,8 @@ __device__ __forceinline__ void flat_walk(const RowArgs &a, int w) {
             }
         }
     }
-    if (n > 0) put_row<TC, VEC, Q, LPE, MASK>(a, cur, sub, (cur == first && carried) || cont, acc);
+    if (n > 0)
+        put_row<TC, VEC, Q, LPE, MASK, SCALE>(a, cur, sub, (cur == first && carried) || cont, acc);
 }
 
 // One balance block (or one reside column when unbalanced) per wave: the

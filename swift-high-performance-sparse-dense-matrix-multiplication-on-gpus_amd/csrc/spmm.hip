@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_tiles_row(RowArgs a) {
 // FUSE: blocks from a.walk_blocks on run the reside blocks, one wave each,
 // and the walk stores the rows they add into with atomics.
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF,
-          bool FUSE = false>
+          bool FUSE = false, int SLOTS = 0>
 __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
     if constexpr (FUSE) {
         if (blockIdx.x >= (unsigned)a.walk_blocks) {
@@ -94,7 +94,31 @@ __global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
     // wave-uniform segment id, kept in a scalar register
     const int sid = __builtin_amdgcn_readfirstlane((int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6));
     if (sid >= a.nseg) return;
-    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF, false, FUSE>(a, sid);
+    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF, false, FUSE, SLOTS>(a, sid);
+}
+
+// The row-view walk with a.slots slot rows per step where the autotuner
+// chose a count other than the rule that the shape can take (buffer loads
+// only), else with the rule.
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF, bool FUSE>
+static void launch_view(const RowArgs &a, dim3 grid, hipStream_t s) {
+    if constexpr (BUF && !MASK) {
+        constexpr int RULE = rule_slots(LPE, Q, (int)sizeof(TB));
+#define SWIFT_SLOTS_CASE(N_)                                                              \
+        if constexpr (slots_ok(LPE, Q, N_) && RULE != N_) {                              \
+            if (a.slots == N_) {                                                         \
+                hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, BUF, FUSE, N_>), \
+                                   grid, dim3(256), 0, s, a);                            \
+                return;                                                                  \
+            }                                                                            \
+        }
+        SWIFT_SLOTS_CASE(4)
+        SWIFT_SLOTS_CASE(8)
+        SWIFT_SLOTS_CASE(16)
+#undef SWIFT_SLOTS_CASE
+    }
+    hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, BUF, FUSE>), grid, dim3(256),
+                       0, s, a);
 }
 
 // Lane shapes whose walk keeps its register count with the reside blocks and
@@ -466,22 +490,18 @@ static void launch_row_shape(const RowArgs &a, RowKind kind, hipStream_t s) {
                 // the reside blocks run as extra blocks of this launch
                 const dim3 grid((unsigned)(w.walk_blocks + ceil_div((size_t)a.nblk * 64, 256)));
                 if (a.bstride)
-                    hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, true, true>),
-                                       grid, dim3(256), 0, s, w);
+                    launch_view<TV, TB, TC, VEC, Q, LPE, MASK, true, true>(w, grid, s);
                 else
-                    hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, false, true>),
-                                       grid, dim3(256), 0, s, w);
+                    launch_view<TV, TB, TC, VEC, Q, LPE, MASK, false, true>(w, grid, s);
                 return;
             }
         }
         if (!a.nseg) return;
         const dim3 grid((unsigned)w.walk_blocks);
         if (a.bstride)
-            hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, true>), grid,
-                               dim3(256), 0, s, w);
+            launch_view<TV, TB, TC, VEC, Q, LPE, MASK, true, false>(w, grid, s);
         else
-            hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, false>), grid,
-                               dim3(256), 0, s, w);
+            launch_view<TV, TB, TC, VEC, Q, LPE, MASK, false, false>(w, grid, s);
         return;
     }
     const unsigned items = kind == ROW_TILES ? (unsigned)a.nwork : (unsigned)a.nblk;
@@ -566,6 +586,11 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     constexpr int V = Vec16<TB>::N;
     const bool aligned = (K % V) == 0 && ((uintptr_t)B % 16) == 0 && ((uintptr_t)C % 16) == 0;
     const auto slices = plan_row_slices(K, V, aligned);
+    // the autotuner's choice for this K and element size (DEFAULT only)
+    const bool tuned = kernel == SWIFT_KERNEL_DEFAULT && M->tuned_K == K &&
+                       M->tuned_elem == (int)sizeof(TB);
+    a.slots = tuned ? M->tuned_slots : 0;
+    if (tuned && M->tuned_kernel == SWIFT_KERNEL_WINDOW) kernel = SWIFT_KERNEL_WINDOW;
     const bool flat = kernel == SWIFT_KERNEL_FLAT;
     const bool winw = kernel == SWIFT_KERNEL_WINDOW;
     const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW || flat ||

@@ -161,3 +161,70 @@ def test_autotune_single(swift, oracle, cuda):
     C64, absC = oracle.spmm_f64(m, n, rp, ci, v32.astype(np.float64), b.astype(np.float64))
     assert gpu_common.rel_err(C.cpu().numpy(), C64, absC) <= TOL
     M.close()
+
+
+# lane shape of the first K slice -> slot-row counts the autotuner may pick
+# (0 = the rule: 8 for 32/64-lane groups, 4 below; 16 only where a step still
+# fits one staged batch of 64 entries)
+SLOT_CHOICES = {8: {0, 8}, 16: {0, 8, 16}, 32: {0, 4, 16}, 64: {0, 4, 16}}
+
+
+@pytest.mark.parametrize("dt,K,lpe", [("f32", 256, 64), ("bf16", 64, 16), ("f32", 100, 16),
+                                      ("f64", 32, 16), ("f32", 128, 32)])
+def test_autotune_schedule_applied(swift, oracle, cuda, dt, K, lpe):
+    """swift_autotune over segment size x slot rows x walk on the benchmark
+    structure at 1/10 scale: the choice is reported by swift_info, taken by
+    the DEFAULT product of that K (before and after another K's product) and
+    the product matches the oracle."""
+    import torch
+    print("case autotune", dt, K, flush=True)
+    d = swift.generate(2, scale=0.1, seed=0)
+    m, n, rp, ci = d["m"], d["n"], d["rowptr"], d["colidx"]
+    f64 = dt == "f64"
+    v = d["val"].astype(np.float64 if f64 else np.float32)
+    M = swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float64 if f64 else torch.float32)
+    b = np.random.default_rng(K).uniform(-1, 1, (n, K)).astype(v.dtype)
+    B = torch.tensor(b, device="cuda")
+    code = swift.F64 if f64 else swift.F32
+    if dt == "bf16":
+        M.prepare_values(swift.BF16)
+        B = swift.convert_dense(B, swift.BF16)
+        code = swift.BF16
+    C = torch.empty((m, K), device="cuda", dtype=torch.float64 if f64 else torch.float32)
+    W = M.autotune(B, C)
+    inf = M.info()
+    print("tuned", W, inf.tuned_K, inf.tuned_slots, inf.tuned_kernel, flush=True)
+    assert inf.segment_items == W and W in (64, 128, 256, 512) and inf.tuned_K == K
+    assert inf.tuned_kernel in (swift.KERNEL_DEFAULT, swift.KERNEL_WINDOW)
+    assert inf.tuned_slots in SLOT_CHOICES[lpe]
+    if inf.tuned_kernel == swift.KERNEL_WINDOW:
+        assert inf.tuned_slots == 0
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, v.astype(np.float64), b.astype(np.float64))
+    tol = 1e-12 if f64 else 1e-2 if dt == "bf16" else TOL
+    out = torch.full_like(C, float("nan"))
+    M.spmm(B, out=out, dtype=code)
+    assert gpu_common.rel_err(out.cpu().numpy(), C64, absC) <= tol
+    # another K: the rule applies there, and the tuned K keeps its choice
+    M.spmm(B[:, :K // 2].contiguous(), dtype=code)
+    out.fill_(float("nan"))
+    M.spmm(B, out=out, dtype=code)
+    assert gpu_common.rel_err(out.cpu().numpy(), C64, absC) <= tol
+    M.close()
+
+
+def test_batch_schedules(swift, oracle, cuda):
+    """Every batched matrix runs the schedule its key was tuned to: a segment
+    size and, per K, the rule or the batched walk's other slot-row count."""
+    items = make_batch(swift, 16, seed=5)
+    batch = swift.Batch([it[0] for it in items], [it[1] for it in items])
+    for M, K, _ in items:
+        inf = M.info()
+        assert inf.segment_items in (64, 128, 256, 512)
+        assert inf.tuned_K == K and inf.tuned_kernel == swift.KERNEL_DEFAULT
+        assert inf.tuned_slots in ({0, 8} if K <= 64 else {0, 4})
+    Bs, Bn, Cs = operands(items, 4)
+    batch.spmm(Bs, Cs, graph=True)
+    check_all(oracle, items, Bn, Cs)
+    batch.close()
+    for it in items:
+        it[0].close()
