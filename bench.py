@@ -148,8 +148,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE baseline")
     ap.add_argument("--extras", action="store_true", help="also time the other forms")
-    ap.add_argument("--batch", type=int, default=0,
-                    help="also time the batched form over this many generated matrices")
+    ap.add_argument("--batch", type=int, default=512,
+                    help="matrices of the batched form in the configuration table (0: skip)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the configuration table (the other benchmark configurations)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 256 MiB device copy (counter calibration)")
     ap.add_argument("--kernel", default="DEFAULT",
@@ -285,11 +287,24 @@ def main():
         log("forms %.1fs: %s" % (time.time() - t0, out["forms"]))
     if args.extras:
         out["extras"] = extras(S, torch, M, d, B, K)
+    table = args.part == "all" and not bf16 and not args.no_configs and K == 128
+    if table:
+        t0 = time.time()
+        out["configs"] = config_table(S, torch, M, d, B, ms_step)
+        log("configs on the 10^6-row matrix %.1fs" % (time.time() - t0))
     M.close()
     del B, C
     torch.cuda.empty_cache()
+    if table:
+        t0 = time.time()
+        out["configs"].update(gnn_config(S, torch, args.scale))
+        log("graph configuration %.1fs" % (time.time() - t0))
+        torch.cuda.empty_cache()
     if args.batch:
-        out["batched"] = batched(S, torch, args.batch, args.scale)
+        t0 = time.time()
+        res = batched(S, torch, args.batch, args.scale)
+        out.setdefault("configs", {})["C5 batched %d matrices" % args.batch] = res
+        log("batched %.1fs" % (time.time() - t0))
         torch.cuda.empty_cache()
     if not args.no_vendor and args.part == "all":
         t0 = time.time()
@@ -352,6 +367,128 @@ def headline_forms(S, torch, M, d, B, K, f32_ms):
         "fwd_bwd": {"fwd_ms": round(f32_ms, 4), "bwd_fused_ms": round(bw, 4),
                     "fwd_bwd_over_fwd": round((f32_ms + bw) / f32_ms, 3)},
     }
+
+
+def entry(ms, flops, alg):
+    """One configuration: time per product, GFLOP/s, algorithmic bytes and
+    their fraction of the HBM roofline."""
+    return {"ms": round(ms, 4), "gflops": round(flops / ms / 1e6, 1), "algorithmic_bytes": alg,
+            "roofline_frac": round(alg / (ms * 1e-3) / HBM_PEAK, 4)}
+
+
+def config_table(S, torch, M, d, B, f32_ms):
+    """The configurations other than the headline step on the 10^6-row
+    matrix: FP32 column-major and K=32, FP64 K=128/32, BF16 storage at K=256
+    and F16 storage with per-column scales at K=128/256 (each with its error
+    against FP64), and K=256 after the autotuner.  Times are whole products
+    (HIP events over 10-20 calls)."""
+    m, n, nnz = d["m"], d["n"], d["nnz"]
+    res = {}
+    C = torch.empty((m, 128), device="cuda")
+    Bt = B.t().contiguous()
+    Ct = torch.empty((128, m), device="cuda")
+    ms = time_call(torch, lambda: M.spmm(Bt, out=Ct, layout=S.COL_MAJOR), reps=10)
+    res["C2 f32 K=128 col-major"] = entry(ms, 2.0 * nnz * 128, algorithmic_bytes(m, n, nnz, 128))
+    del Bt, Ct
+    B32 = B[:, :32].contiguous()
+    C32 = torch.empty((m, 32), device="cuda")
+    ms = time_call(torch, lambda: M.spmm(B32, out=C32), reps=20)
+    res["C2 f32 K=32"] = entry(ms, 2.0 * nnz * 32, algorithmic_bytes(m, n, nnz, 32))
+    del B32, C32
+    torch.cuda.empty_cache()
+    # FP64 build of the same matrix
+    M64 = S.SwiftMatrix(m, n, d["rowptr"], d["colidx"], d["val"], dtype=torch.float64)
+    for K in (128, 32):
+        Bd = B[:, :K].double().contiguous()
+        Cd = torch.empty((m, K), device="cuda", dtype=torch.float64)
+        ms = time_call(torch, lambda: M64.spmm(Bd, out=Cd), reps=10)
+        res["C2 f64 K=%d" % K] = entry(ms, 2.0 * nnz * K,
+                                       algorithmic_bytes(m, n, nnz, K, s_a=8, s_b=8, s_c=8))
+        del Bd, Cd
+    M64.close()
+    torch.cuda.empty_cache()
+    # 16-bit storage at K = 256 (K = 128 BF16 is in "forms"), F16 with scales
+    g = torch.Generator(device="cuda").manual_seed(1)
+    B256 = torch.rand((n, 256), device="cuda", generator=g) * 2 - 1
+    C256 = torch.empty((m, 256), device="cuda")
+    M.prepare_values(S.BF16)
+    Bh = S.convert_dense(B256, S.BF16)
+    ms = time_call(torch, lambda: M.spmm(Bh, out=C256, dtype=S.BF16), reps=10)
+    e = entry(ms, 2.0 * nnz * 256, algorithmic_bytes(m, n, nnz, 256, s_a=2, s_b=2))
+    e["error"] = lowp_error(torch, d, B256, C256)
+    ms_f32 = time_call(torch, lambda: M.spmm(B256, out=C256), reps=10)
+    e["f32_ms"] = round(ms_f32, 4)
+    e["speedup_vs_f32"] = round(ms_f32 / ms, 3)
+    res["C3 bf16 K=256"] = e
+    del Bh
+    M.prepare_values(S.F16)
+    for K, Bk, Ck in ((128, B, C), (256, B256, C256)):
+        Bf, sc = S.convert_dense(Bk, S.F16, col_scale=True)
+        ms = time_call(torch, lambda: M.spmm_scaled(Bf, sc, out=Ck), reps=10)
+        e = entry(ms, 2.0 * nnz * K, algorithmic_bytes(m, n, nnz, K, s_a=2, s_b=2))
+        e["error"] = lowp_error(torch, d, Bk, Ck)
+        res["C3 f16 column-scaled K=%d" % K] = e
+        del Bf, sc
+    # the autotuner's schedules at K = 256 (they pin the matrix's segment
+    # size, so they come last)
+    M.prepare_values(S.BF16)
+    Bh = S.convert_dense(B256, S.BF16)
+    for name, Bk, code, sa, base in (("C3 bf16 K=256 autotuned", Bh, S.BF16, 2,
+                                      res["C3 bf16 K=256"]["ms"]),
+                                     ("C2 f32 K=256 autotuned", B256, S.F32, 4, ms_f32)):
+        M.autotune(Bk, C256)
+        inf = M.info()
+        ms = time_call(torch, lambda: M.spmm(Bk, out=C256, dtype=code), reps=10)
+        res[name] = dict(entry(ms, 2.0 * nnz * 256, algorithmic_bytes(m, n, nnz, 256, s_a=sa,
+                                                                        s_b=sa)),
+                         default_ms=round(base, 4),
+                         schedule={"segment_items": inf.segment_items, "slots": inf.tuned_slots,
+                                   "kernel": inf.tuned_kernel})
+    del Bh
+    del B256, C256, C
+    torch.cuda.empty_cache()
+    log("configs: %s" % res)
+    return res
+
+
+def backward_bytes(m, n, nnz, K, s=4):
+    """Fused backward over A^T: per entry its column, value, CSR source index
+    and dvals once (4 + s + 4 + s bytes), A^T's row pointer, dC gathered once
+    (m*K), B and dB once each (n*K)."""
+    return nnz * (8 + 2 * s) + 4 * (n + 1) + m * K * s + 2 * n * K * s
+
+
+def gnn_config(S, torch, scale):
+    """The differentiation configuration: the symmetric 200,000-node power-law
+    graph with D^-1/2 (A+I) D^-1/2 values, K = 128, FP32 forward, BF16
+    forward, fused backward (dB = A^T dC and dvals = SDDMM(dC, B)) and the
+    forward + backward to forward ratio."""
+    d = S.generate(4, scale=scale, seed=0)
+    m, n, nnz = d["m"], d["n"], d["nnz"]
+    K = 128
+    M = S.SwiftMatrix(m, n, d["rowptr"], d["colidx"], d["val"].astype(np.float32))
+    g = torch.Generator(device="cuda").manual_seed(2)
+    B = torch.randn((n, K), device="cuda", generator=g)
+    dC = torch.randn((m, K), device="cuda", generator=g)
+    C = torch.empty((m, K), device="cuda")
+    fwd = min(time_call(torch, lambda: M.spmm(B, out=C), reps=20) for _ in range(3))
+    M.backward_fused(dC, B)  # builds the transposed layout once
+    bwd = time_call(torch, lambda: M.backward_fused(dC, B), reps=20)
+    M.prepare_values(S.BF16)
+    Bh = S.convert_dense(B, S.BF16)
+    fbf = time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16), reps=20)
+    res = {
+        "C4 graph f32 forward": dict(entry(fwd, 2.0 * nnz * K, algorithmic_bytes(m, n, nnz, K)),
+                                     nodes=m, nnz=nnz),
+        "C4 graph bf16 forward": entry(fbf, 2.0 * nnz * K,
+                                       algorithmic_bytes(m, n, nnz, K, s_a=2, s_b=2)),
+        "C4 graph fused backward": entry(bwd, 4.0 * nnz * K, backward_bytes(m, n, nnz, K)),
+        "C4 graph fwd+bwd / fwd": round((fwd + bwd) / fwd, 3),
+    }
+    M.close()
+    del B, dC, C, Bh
+    log("graph: %s" % res)
+    return res
 
 
 def extras(S, torch, M, d, B, K):
@@ -453,8 +590,10 @@ def batched(S, torch, count, scale=1.0, reps=10):
     nnz_total = 0
     alg = 0
     g = torch.Generator(device="cuda").manual_seed(0)
-    for i in range(count):
-        d = S.generate(5, scale=scale, seed=0, index=i)
+    import concurrent.futures as cf
+    with cf.ThreadPoolExecutor(8) as ex:  # the generator releases the GIL
+        gen = list(ex.map(lambda i: S.generate(5, scale=scale, seed=0, index=i), range(count)))
+    for d in gen:
         M = S.SwiftMatrix(d["m"], d["n"], d["rowptr"], d["colidx"], d["val"].astype(np.float32),
                           dtype=torch.float32)
         K = d["k"]

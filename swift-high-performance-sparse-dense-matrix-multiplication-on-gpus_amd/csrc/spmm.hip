@@ -76,12 +76,23 @@ __global__ __launch_bounds__(256) void k_tiles_row(RowArgs a) {
     if (cur >= 0) flush_strided<TC, VEC, Q, LPE, MASK>(a, cur, sub, acc);
 }
 
+// Waves per SIMD the walk may reach.  The FP32 walk of 32-lane groups (K =
+// 128) fits 64 VGPRs, i.e. 8 waves per SIMD, and runs faster at 7: on the
+// 10^6-row matrix 0.829-0.831 ms at 8 against 0.821-0.823 ms at 7 (the
+// 200k-node graph: equal).  Other shapes keep the limit of their register
+// count.
+template <class TB, int Q, int LPE>
+constexpr int walk_max_waves() {
+    return (sizeof(TB) == 4 && Q == 1 && LPE == 32) ? 7 : 8;
+}
+
 // Thin kernels over the device functions of rowview.h.
 // FUSE: blocks from a.walk_blocks on run the reside blocks, one wave each,
 // and the walk stores the rows they add into with atomics.
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF,
           bool FUSE = false, int SLOTS = 0>
-__global__ __launch_bounds__(256) void k_rowview_row(RowArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, walk_max_waves<TB, Q, LPE>())))
+void k_rowview_row(RowArgs a) {
     if constexpr (FUSE) {
         if (blockIdx.x >= (unsigned)a.walk_blocks) {
             const int blk =
