@@ -105,10 +105,12 @@ typedef struct {
      * merge-path segment set by swift_set_segment_items or the autotuner (0:
      * the default tables); for products of tuned_K columns (0: none tuned)
      * the autotuner's walk (SWIFT_KERNEL_DEFAULT = row-view walk,
-     * SWIFT_KERNEL_WINDOW = window walk) and slot rows per step (0: the
-     * rule). */
+     * SWIFT_KERNEL_WINDOW = window walk), slot rows per step (0: the
+     * rule) and the widest K slice of one launch (0: no cap; a cap runs K as
+     * several slices of narrower lane groups). */
     int segment_items, tuned_K, tuned_slots;
     swift_kernel tuned_kernel;
+    int tuned_cols;
 } swift_info;
 
 /* ------------------------------------------------------------------ host */
@@ -382,7 +384,8 @@ void swift_batch_destroy(swift_batch *batch);
  * operands (C is overwritten) for every candidate schedule and keep the
  * fastest for products of this K and dtype: segment sizes {64, 128, 256,
  * 512} x the slot rows per step the product's lane shape can take (4, 8,
- * 16), and the window walk (row-major).  best_items receives the segment
+ * 16), the lane shape of half the width (K in two slices), and the window
+ * walk (row-major).  best_items receives the segment
  * size; swift_info reports the rest.  Has no counterpart in the reference,
  * whose tiling constants are fixed at compile time
  * (test/k-32/FP32/main.cu:21-23). */

@@ -225,6 +225,7 @@ extern "C" int swift_get_info(const swift_matrix *M, swift_info *o) {
     o->tuned_K = M->tuned_K;
     o->tuned_slots = M->tuned_slots;
     o->tuned_kernel = M->tuned_kernel;
+    o->tuned_cols = M->tuned_cols;
     return SWIFT_OK;
 }
 

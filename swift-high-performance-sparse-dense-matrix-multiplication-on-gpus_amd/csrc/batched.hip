@@ -151,6 +151,7 @@ using TuneKey = std::tuple<int, int, int, int>;
 struct TuneChoice {
     int W = 128, slots = 0;
     swift_kernel kernel = SWIFT_KERNEL_DEFAULT;
+    int cols = 0;  // widest K slice of one launch (0: no cap)
 };
 static std::map<TuneKey, TuneChoice> g_tuned;
 static std::mutex g_tune_mu;
@@ -172,32 +173,38 @@ static int apply_choice(swift_matrix *M, int K, size_t elem, const TuneChoice &c
     M->tuned_elem = (int)elem;
     M->tuned_slots = c.slots;
     M->tuned_kernel = c.kernel;
+    M->tuned_cols = c.cols;
     return SWIFT_OK;
 }
 
 // Times the SWIFT_KERNEL_DEFAULT product of M on the given operands for every
 // candidate schedule and keeps the fastest (median of 3 after one warm-up).
 // Candidates: segment sizes {64, 128, 256, 512} x the slot-row counts of the
-// product's lane shape (slot_candidates), plus the window walk (LDS-staged
-// column windows) at each size.  for_batch: only what the batched walk runs,
-// the rule and batch_alt_slots, with the row-view walk.
+// product's lane shape (slot_candidates), plus at each size the window walk
+// (LDS-staged column windows) and the lane shape of half the width
+// (width_candidates: K in two slices of narrower lane groups).  for_batch:
+// only what the batched walk runs, the rule and batch_alt_slots, with the
+// row-view walk.
 int autotune_impl(swift_matrix *M, int K, swift_dtype dtype, swift_layout layout, const void *B,
                   void *C, hipStream_t s, bool for_batch, TuneChoice *best_out) {
     static const int sizes[] = {64, 128, 256, 512};
     const size_t elem = value_size(dtype);
     std::vector<TuneChoice> cands;
-    std::vector<int> slots;
+    std::vector<int> slots, widths{0};
     if (for_batch) {
         slots = {0, batch_alt_slots(K)};
     } else {
         const int vec = vec_of(elem);
         const bool aligned = K % vec == 0 && (((uintptr_t)B | (uintptr_t)C) % 16) == 0;
         slots = slot_candidates(K, elem, aligned);
+        widths = width_candidates(K, elem, aligned);
     }
     for (int W : sizes) {
-        for (int n : slots) cands.push_back({W, n, SWIFT_KERNEL_DEFAULT});
+        for (int n : slots) cands.push_back({W, n, SWIFT_KERNEL_DEFAULT, 0});
+        for (int w : widths)
+            if (w) cands.push_back({W, 0, SWIFT_KERNEL_DEFAULT, w});
         if (!for_batch && layout == SWIFT_ROW_MAJOR)
-            cands.push_back({W, 0, SWIFT_KERNEL_WINDOW});
+            cands.push_back({W, 0, SWIFT_KERNEL_WINDOW, 0});
     }
     hipEvent_t e0, e1;
     SWIFT_HIP(hipEventCreate(&e0));
@@ -363,7 +370,7 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
             }
             for (int i = 0; i < count && rc == SWIFT_OK; i++)
                 rc = apply_choice(mats[i], Ks[i], sizeof(float),
-                                  TuneChoice{b->W[i], b->slots[i], SWIFT_KERNEL_DEFAULT}, s);
+                                  TuneChoice{b->W[i], b->slots[i], SWIFT_KERNEL_DEFAULT, 0}, s);
         }
         if (rc == SWIFT_OK) {
             // flattened work lists (host) -> device

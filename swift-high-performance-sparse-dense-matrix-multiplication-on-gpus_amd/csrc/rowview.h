@@ -262,17 +262,21 @@ struct RowTarget {
 
 // Splits [0, K) into slices handled by one kernel each: the largest
 // 16-byte-vector shape that fits, then masked scalar slices of <= 64.  Shared
-// by the forward and backward walks.
-inline std::vector<std::pair<int, RowTarget>> plan_row_slices(int K, int vec, bool aligned) {
+// by the forward and backward walks.  max_cols > 0 caps the columns of a
+// full-vector slice (a narrower lane group per entry, one more pass over A per
+// extra slice; each pass gathers from a narrower slice of B).
+inline std::vector<std::pair<int, RowTarget>> plan_row_slices(int K, int vec, bool aligned,
+                                                              int max_cols = 0) {
     std::vector<std::pair<int, RowTarget>> out;
     int k = 0;
+    auto fits = [&](int cols) { return max_cols <= 0 || cols <= max_cols; };
     if (aligned) {
         static const int qs[] = {4, 2, 1};
         while (K - k >= vec * 4) {
             int rem = K - k;
             bool done = false;
             for (int q : qs) {
-                if (rem >= vec * 64 * q) {
+                if (rem >= vec * 64 * q && fits(vec * 64 * q)) {
                     out.push_back({k, {vec, q, 64, false}});
                     k += vec * 64 * q;
                     done = true;
@@ -282,7 +286,7 @@ inline std::vector<std::pair<int, RowTarget>> plan_row_slices(int K, int vec, bo
             if (done) continue;
             for (int lpe = 32; lpe >= 4; lpe >>= 1) {
                 if (lpe == 4 && vec != 8) break;
-                if (rem >= vec * lpe) {
+                if (rem >= vec * lpe && fits(vec * lpe)) {
                     out.push_back({k, {vec, 1, lpe, false}});
                     k += vec * lpe;
                     done = true;
@@ -583,6 +587,22 @@ constexpr bool slots_ok(int lpe, int q, int slots) {
 
 // Elements per lane load for B elements of elem bytes (Vec16 on the host).
 inline int vec_of(size_t elem) { return elem == 8 ? 2 : elem == 4 ? 4 : SWIFT_NARROW_VEC; }
+
+// Column caps worth timing for a product of K columns (0: none): half the
+// width of the first full-vector slice, when that still is a full-vector
+// shape of at least 8 lanes.
+inline std::vector<int> width_candidates(int K, size_t elem, bool aligned) {
+    std::vector<int> out{0};
+    const int vec = vec_of(elem);
+    for (auto &sl : plan_row_slices(K, vec, aligned)) {
+        const RowTarget &t = sl.second;
+        if (t.mask) break;
+        const int cols = vec * t.lpe * t.q;
+        if (cols / 2 >= vec * 8) out.push_back(cols / 2);
+        break;
+    }
+    return out;
+}
 
 // Slot-row counts worth timing for a product of K columns with elem-byte B
 // elements: 0 (the rule) and every other count the first full-vector slice's

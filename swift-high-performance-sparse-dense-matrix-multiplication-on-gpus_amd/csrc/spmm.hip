@@ -596,10 +596,10 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.gscale = gscale;
     constexpr int V = Vec16<TB>::N;
     const bool aligned = (K % V) == 0 && ((uintptr_t)B % 16) == 0 && ((uintptr_t)C % 16) == 0;
-    const auto slices = plan_row_slices(K, V, aligned);
     // the autotuner's choice for this K and element size (DEFAULT only)
     const bool tuned = kernel == SWIFT_KERNEL_DEFAULT && M->tuned_K == K &&
                        M->tuned_elem == (int)sizeof(TB);
+    const auto slices = plan_row_slices(K, V, aligned, tuned ? M->tuned_cols : 0);
     a.slots = tuned ? M->tuned_slots : 0;
     if (tuned && M->tuned_kernel == SWIFT_KERNEL_WINDOW) kernel = SWIFT_KERNEL_WINDOW;
     const bool flat = kernel == SWIFT_KERNEL_FLAT;

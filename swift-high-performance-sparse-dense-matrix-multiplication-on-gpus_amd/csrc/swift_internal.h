@@ -205,9 +205,10 @@ struct swift_matrix {
     int user_items = 0;
     // autotuner choice (swift_autotune, swift_batch_create) for products of
     // tuned_K columns with tuned_elem-byte B elements: the walk that
-    // SWIFT_KERNEL_DEFAULT runs (the row-view walk or the window walk) and
-    // the row-view walk's slot rows per step (0: the rule)
-    int tuned_K = 0, tuned_elem = 0, tuned_slots = 0;
+    // SWIFT_KERNEL_DEFAULT runs (the row-view walk or the window walk), the
+    // row-view walk's slot rows per step (0: the rule) and the widest K slice
+    // of one launch (0: no cap)
+    int tuned_K = 0, tuned_elem = 0, tuned_slots = 0, tuned_cols = 0;
     swift_kernel tuned_kernel = SWIFT_KERNEL_DEFAULT;
 
     // build phase times (ms): column sort (csr->csc + ColSort), blocking

@@ -46,7 +46,8 @@ class Info(C.Structure):
                            "colorblk", "blkstart", "blkstop", "blkptr", "val_position",
                            "val_offset", "rowsorted", "sortsrc", "reside_src")] + [
         ("build_ms", C.c_float * 4), ("rowview_nnz", C.c_int), ("segment_items", C.c_int),
-        ("tuned_K", C.c_int), ("tuned_slots", C.c_int), ("tuned_kernel", C.c_int)]
+        ("tuned_K", C.c_int), ("tuned_slots", C.c_int), ("tuned_kernel", C.c_int),
+        ("tuned_cols", C.c_int)]
 
 
 class Stats(C.Structure):

@@ -169,10 +169,11 @@ def test_autotune_single(swift, oracle, cuda):
 SLOT_CHOICES = {8: {0, 8}, 16: {0, 8, 16}, 32: {0, 4, 16}, 64: {0, 4, 16}}
 
 
-@pytest.mark.parametrize("dt,K,lpe", [("f32", 256, 64), ("bf16", 64, 16), ("f32", 100, 16),
-                                      ("f64", 32, 16), ("f32", 128, 32)])
-def test_autotune_schedule_applied(swift, oracle, cuda, dt, K, lpe):
-    """swift_autotune over segment size x slot rows x walk on the benchmark
+@pytest.mark.parametrize("dt,K,lpe,half", [("f32", 256, 64, 128), ("bf16", 64, 16, 32),
+                                           ("f32", 100, 16, 32), ("f64", 32, 16, 16),
+                                           ("f32", 128, 32, 64)])
+def test_autotune_schedule_applied(swift, oracle, cuda, dt, K, lpe, half):
+    """swift_autotune over segment size x slot rows x lane width x walk on the benchmark
     structure at 1/10 scale: the choice is reported by swift_info, taken by
     the DEFAULT product of that K (before and after another K's product) and
     the product matches the oracle."""
@@ -193,11 +194,13 @@ def test_autotune_schedule_applied(swift, oracle, cuda, dt, K, lpe):
     C = torch.empty((m, K), device="cuda", dtype=torch.float64 if f64 else torch.float32)
     W = M.autotune(B, C)
     inf = M.info()
-    print("tuned", W, inf.tuned_K, inf.tuned_slots, inf.tuned_kernel, flush=True)
+    print("tuned", W, inf.tuned_K, inf.tuned_slots, inf.tuned_kernel, inf.tuned_cols, flush=True)
     assert inf.segment_items == W and W in (64, 128, 256, 512) and inf.tuned_K == K
     assert inf.tuned_kernel in (swift.KERNEL_DEFAULT, swift.KERNEL_WINDOW)
     assert inf.tuned_slots in SLOT_CHOICES[lpe]
-    if inf.tuned_kernel == swift.KERNEL_WINDOW:
+    # a width cap halves the first slice's lane group (K in more launches)
+    assert inf.tuned_cols in (0, half)
+    if inf.tuned_kernel == swift.KERNEL_WINDOW or inf.tuned_cols:
         assert inf.tuned_slots == 0
     C64, absC = oracle.spmm_f64(m, n, rp, ci, v.astype(np.float64), b.astype(np.float64))
     tol = 1e-12 if f64 else 1e-2 if dt == "bf16" else TOL
