@@ -341,7 +341,10 @@ int swift_sddmm(const swift_matrix *A, int K, swift_dtype dtype,
 /* Fused backward in one pass over the transposed Swift layout AT (from
  * swift_build_transpose): dB = A^T*dC (n x K) and dvals = SDDMM(dC, B) in
  * A's CSR order; every dC row is gathered once per nonzero and feeds both.
- * Either output may be NULL.  Row-major, original column order, F32/F64. */
+ * Either output may be NULL.  Row-major, original column order, F32/F64.
+ * dvals are written in AT's walk order into a workspace of nnz values held
+ * by AT, then gathered into CSR order: calls with dvals on one AT must not
+ * overlap in time (issue them on one stream). */
 int swift_backward_fused(const swift_matrix *AT, int K, swift_dtype dtype,
                          const void *d_dC, const void *d_B, void *d_dB,
                          void *d_dvals, swift_stream_t stream);

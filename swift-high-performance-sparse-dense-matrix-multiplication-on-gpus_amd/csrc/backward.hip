@@ -42,6 +42,53 @@ __global__ void k_compose(int *__restrict__ a, const int *__restrict__ map, size
     if (i < n) a[i] = map[a[i]];
 }
 
+// out[src[i]] = i: the inverse of a permutation of 0..n-1.
+__global__ void k_invert(const unsigned *__restrict__ src, size_t n, int *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[src[i]] = (int)i;
+}
+
+// keys[i] = src[i] >> lg (src: signed positions or unsigned walk indices)
+template <class I>
+__global__ void k_shift_keys(const I *__restrict__ src, size_t n, int lg,
+                             unsigned *__restrict__ keys) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = (unsigned)src[i] >> lg;
+}
+
+// off[s] = CSR position, inside its bucket, of the entry in bucket-order slot s.
+__global__ void k_bucket_offsets(const unsigned *__restrict__ walk, const int *__restrict__ src,
+                                 size_t n, int lg, unsigned short *__restrict__ off) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) off[i] = (unsigned short)(src[walk[i]] & ((1 << lg) - 1));
+}
+
+// For position p of the (chunk, bucket, walk) order, whose entry is walk
+// position e = order[p]: rank[e] = p inside the chunk, dst[p] = slot of e in
+// bucket order.
+__global__ void k_chunk_plan(const unsigned *__restrict__ order, const int *__restrict__ slot,
+                             size_t n, int lg, unsigned short *__restrict__ rank,
+                             int *__restrict__ dst) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned e = order[i];
+    rank[e] = (unsigned short)(i & ((1u << lg) - 1));
+    dst[i] = slot[e];
+}
+
+// Sizes (log2 of values) of the dvals reordering: a bucket (pass 2) holds
+// 128 KiB of values, a chunk (pass 1) 64 KiB, so that two pass-1 workgroups
+// share a CU; each is reordered in LDS by one workgroup.
+template <class T> struct DvPlan;
+template <> struct DvPlan<float> { static constexpr int BUCKET = 15, CHUNK = 13; };
+template <> struct DvPlan<double> { static constexpr int BUCKET = 14, CHUNK = 13; };
+static int dv_bucket_log(swift_dtype t) {
+    return t == SWIFT_F64 ? DvPlan<double>::BUCKET : DvPlan<float>::BUCKET;
+}
+static int dv_chunk_log(swift_dtype t) {
+    return t == SWIFT_F64 ? DvPlan<double>::CHUNK : DvPlan<float>::CHUNK;
+}
+
 static unsigned g1(size_t n) { return (unsigned)std::max<size_t>(1, (n + 255) / 256); }
 
 template <class T>
@@ -134,6 +181,63 @@ int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t 
     if (T->rv_nnz)
         hipLaunchKernelGGL(k_compose, dim3(g1(T->rv_nnz)), dim3(256), 0, s, T->rv_src,
                            cscsrc.as<int>(), (size_t)T->rv_nnz);
+    // The row view holds every entry once, so rv_src is a permutation of A's
+    // CSR positions.  The fused backward writes dvals in walk order
+    // (coalesced) and moves them into CSR order in two passes: walk order ->
+    // buckets of CSR positions (a stable sort of the walk positions by
+    // bucket), then each bucket reordered in LDS.
+    if (T->rv_nnz == T->nnz && T->nnz) {
+        const size_t n = (size_t)T->nnz;
+        const int lg = dv_bucket_log(T->vtype), lc = dv_chunk_log(T->vtype);
+        Buffer keys, kout, border, corder, kalt, valt, slot, tmp;
+        SWIFT_TRY(keys.alloc(4 * n));
+        SWIFT_TRY(kout.alloc(4 * n));
+        SWIFT_TRY(border.alloc(4 * n));
+        SWIFT_TRY(corder.alloc(4 * n));
+        SWIFT_TRY(kalt.alloc(4 * n));
+        SWIFT_TRY(valt.alloc(4 * n));
+        SWIFT_TRY(slot.alloc(4 * n));
+        SWIFT_TRY(tmp.alloc(radix_temp_bytes(n)));
+        // bucket order: walk positions stably sorted by bucket
+        hipLaunchKernelGGL(k_shift_keys<int>, dim3(g1(n)), dim3(256), 0, s, T->rv_src, n, lg,
+                           keys.as<unsigned>());
+        SWIFT_CHECK_LAUNCH();
+        int rc2 = radix_sort_pairs(keys.as<unsigned>(), nullptr, kout.as<unsigned>(),
+                                   border.as<unsigned>(), kalt.as<unsigned>(),
+                                   valt.as<unsigned>(), n, bits_for((unsigned)((n - 1) >> lg)),
+                                   tmp.ptr, s);
+        // (chunk, bucket, walk) order: bucket order stably sorted by chunk
+        if (rc2 == SWIFT_OK) {
+            hipLaunchKernelGGL(k_shift_keys<unsigned>, dim3(g1(n)), dim3(256), 0, s,
+                               border.as<unsigned>(), n, lc, keys.as<unsigned>());
+            SWIFT_CHECK_LAUNCH();
+            rc2 = radix_sort_pairs(keys.as<unsigned>(), border.as<unsigned>(),
+                                   kout.as<unsigned>(), corder.as<unsigned>(),
+                                   kalt.as<unsigned>(), valt.as<unsigned>(), n,
+                                   bits_for((unsigned)((n - 1) >> lc)), tmp.ptr, s);
+        }
+        if (rc2 == SWIFT_OK &&
+            (hipMalloc((void **)&T->dv_rank, 2 * n) != hipSuccess ||
+             hipMalloc((void **)&T->dv_off, 2 * n) != hipSuccess ||
+             hipMalloc((void **)&T->dv_dst, 4 * n) != hipSuccess)) {
+            set_error("swift_build_transpose: hipMalloc(%zu) failed", 8 * n);
+            rc2 = SWIFT_ERR_ALLOC;
+        }
+        if (rc2 != SWIFT_OK) {
+            destroy_matrix(T);
+            return rc2;
+        }
+        T->dv_log = lg;
+        T->dv_clog = lc;
+        hipLaunchKernelGGL(k_invert, dim3(g1(n)), dim3(256), 0, s, border.as<unsigned>(), n,
+                           slot.as<int>());
+        hipLaunchKernelGGL(k_bucket_offsets, dim3(g1(n)), dim3(256), 0, s, border.as<unsigned>(),
+                           T->rv_src, n, lg, T->dv_off);
+        hipLaunchKernelGGL(k_chunk_plan, dim3(g1(n)), dim3(256), 0, s, corder.as<unsigned>(),
+                           slot.as<int>(), n, lc, T->dv_rank, T->dv_dst);
+        SWIFT_CHECK_LAUNCH();
+        SWIFT_HIP(hipStreamSynchronize(s));
+    }
     SWIFT_CHECK_LAUNCH();
     SWIFT_HIP(hipStreamSynchronize(s));
     *out = T;
@@ -266,10 +370,12 @@ __device__ __forceinline__ T multi_group_sum(const T (&d)[N], int sub) {
 }
 
 // dvals of one entry: the first K slice stores its partial dot, later slices
-// add theirs (one lane per entry and launch, launches in stream order).
+// add theirs (one lane per entry and launch, launches in stream order).  The
+// entry's slot is its CSR position, or with WALK its row-view position
+// (consecutive entries, coalesced stores; the caller reorders afterwards).
 template <class T>
-__device__ __forceinline__ void put_dval(const BwArgs &a, int src, T r) {
-    T *dv = static_cast<T *>(a.dvals) + src;
+__device__ __forceinline__ void put_dval(const BwArgs &a, int slot, T r) {
+    T *dv = static_cast<T *>(a.dvals) + slot;
     *dv = a.dv_add ? *dv + r : r;
 }
 
@@ -279,10 +385,10 @@ __device__ __forceinline__ void put_dval(const BwArgs &a, int src, T r) {
 // DB: acc += v * G[c] for the row's dB.  DV: dvals[src] = <G[c], H[row]>, the
 // slots' partial dots summed over the lane group together (lane sub then
 // holds slot sub / (LPE / UNROLL)).
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, int UNROLL, int N>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK, int UNROLL, int N>
 __device__ __forceinline__ void bw_body(const BwArgs &a, __amdgpu_buffer_rsrc_t rsrc, int base,
                                         int j, int nb, int h, int sub, int mc, int ms, T mv,
-                                        const T (&hr)[Q][VEC], T (&acc)[Q][VEC]) {
+                                        const T (&hr)[Q][VEC], T (&acc)[Q][VEC], int pos) {
     constexpr int EPS = 64 / LPE;
     T g[N][Q][VEC];
     T vv[N];
@@ -332,25 +438,25 @@ __device__ __forceinline__ void bw_body(const BwArgs &a, __amdgpu_buffer_rsrc_t 
         }
         const T r = multi_group_sum<T, LPE, UNROLL>(d, sub);
         // slots past the row's end (us >= N among them) write nothing
-        if (sub % LEAD == 0 && j + us * EPS + h < nb) put_dval(a, sv, r);
+        if (sub % LEAD == 0 && j + us * EPS + h < nb) put_dval(a, WALK ? pos + us * EPS : sv, r);
     }
 }
 
 // The body for nu = min(UNROLL, slot rows left) slot rows (wave-uniform).
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, int UNROLL, int N>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK, int UNROLL, int N>
 __device__ __forceinline__ void bw_pick(int nu, const BwArgs &a, __amdgpu_buffer_rsrc_t rsrc,
                                         int base, int j, int nb, int h, int sub, int mc, int ms,
-                                        T mv, const T (&hr)[Q][VEC], T (&acc)[Q][VEC]) {
+                                        T mv, const T (&hr)[Q][VEC], T (&acc)[Q][VEC], int pos) {
     if constexpr (N == 1) {
-        bw_body<T, VEC, Q, LPE, MASK, DB, DV, UNROLL, 1>(a, rsrc, base, j, nb, h, sub, mc, ms, mv,
-                                                         hr, acc);
+        bw_body<T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, 1>(a, rsrc, base, j, nb, h, sub, mc, ms, mv,
+                                                         hr, acc, pos);
     } else {
         if (nu >= N)
-            bw_body<T, VEC, Q, LPE, MASK, DB, DV, UNROLL, N>(a, rsrc, base, j, nb, h, sub, mc, ms,
-                                                             mv, hr, acc);
+            bw_body<T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, N>(a, rsrc, base, j, nb, h, sub, mc, ms,
+                                                             mv, hr, acc, pos);
         else
-            bw_pick<T, VEC, Q, LPE, MASK, DB, DV, UNROLL, N - 1>(nu, a, rsrc, base, j, nb, h, sub,
-                                                                 mc, ms, mv, hr, acc);
+            bw_pick<T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, N - 1>(nu, a, rsrc, base, j, nb, h, sub,
+                                                                 mc, ms, mv, hr, acc, pos);
     }
 }
 
@@ -417,7 +523,7 @@ __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
 #ifndef SWIFT_BW_UNROLL_LONG
 #define SWIFT_BW_UNROLL_LONG 4
 #endif
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool BUF>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK, bool BUF>
 __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     if constexpr (!DB) {
         if (blockIdx.x >= (unsigned)a.walk_blocks) {
@@ -451,7 +557,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     T mv = (T)0;
     if (ebase + lane < e1) {
         mc = ld_stream(a.rv_col + ebase + lane);
-        ms = ld_stream(a.rv_src + ebase + lane);
+        if (DV && !WALK) ms = ld_stream(a.rv_src + ebase + lane);
         mv = ld_stream(val + ebase + lane);
     }
     T acc[Q][VEC], hr[Q][VEC];
@@ -480,7 +586,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 mv = (T)0;
                 if (ebase + lane < e1) {
                     mc = ld_stream(a.rv_col + ebase + lane);
-                    ms = ld_stream(a.rv_src + ebase + lane);
+                    if (DV && !WALK) ms = ld_stream(a.rv_src + ebase + lane);
                     mv = ld_stream(val + ebase + lane);
                 }
             }
@@ -492,8 +598,9 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 int src[UNROLL];
                 if constexpr (BUF) {
                     const int nu = min(UNROLL, (nb - j + EPS - 1) / EPS);
-                    bw_pick<T, VEC, Q, LPE, MASK, DB, DV, UNROLL, UNROLL>(
-                        nu, a, rsrc, (off + j + h) << 2, j, nb, h, sub, mc, ms, mv, hr, acc);
+                    bw_pick<T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, UNROLL>(
+                        nu, a, rsrc, (off + j + h) << 2, j, nb, h, sub, mc, ms, mv, hr, acc,
+                        e + j + h);
                     continue;
                 }
 #pragma unroll
@@ -542,7 +649,8 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                         d[u] = x;
                     }
                     const T r = multi_group_sum<T, LPE, UNROLL>(d, sub);
-                    if (sub % LEAD == 0 && j + us * EPS + h < nb) put_dval(a, sv, r);
+                    if (sub % LEAD == 0 && j + us * EPS + h < nb)
+                        put_dval(a, WALK ? e + j + us * EPS + h : sv, r);
                 }
             }
             e += nb;
@@ -590,6 +698,79 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     }
 }
 
+// dvals from walk order into CSR order in two passes through LDS, one
+// workgroup per chunk of 2^LG values.  Pass 1: chunk c's values are put in
+// LDS by rank (bucket, then walk order) and written to their bucket-order
+// slots, so the chunk's values for one bucket go out as one run of adjacent
+// slots (about 2^LG / buckets of them).  Chunk c + 1's run follows chunk c's
+// in every bucket; neighbouring chunks run on one XCD, so the lines the two
+// runs share are completed in one L2.
+template <class T, int LG>
+__global__ __launch_bounds__(1024) void k_dv_to_buckets(const T *__restrict__ v,
+                                                        const unsigned short *__restrict__ rank,
+                                                        const int *__restrict__ dst, size_t n,
+                                                        T *__restrict__ out) {
+    constexpr int PER = (1 << LG) / 1024;
+    __shared__ T lds[1 << LG];
+    const size_t base = (size_t)xcd_block_bw(blockIdx.x, gridDim.x) << LG;
+    const int cnt = (int)min((size_t)1 << LG, n - base);
+    unsigned short r[PER];
+    T x[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int t = threadIdx.x + i * 1024;
+        r[i] = t < cnt ? rank[base + t] : (unsigned short)0;
+        x[i] = t < cnt ? v[base + t] : (T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++)
+        if ((int)threadIdx.x + i * 1024 < cnt) lds[r[i]] = x[i];
+    // the destinations are loaded before the barrier, so their latency
+    // overlaps the wait for the other waves' LDS stores
+    int d[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int t = threadIdx.x + i * 1024;
+        d[i] = t < cnt ? dst[base + t] : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int t = threadIdx.x + i * 1024;
+        if (t < cnt) out[d[i]] = lds[t];
+    }
+}
+
+// Pass 2, in place: one workgroup per bucket of 2^LG CSR positions; its
+// values (bucket-order slots) go to LDS at their CSR offsets and are written
+// back in order.
+template <class T, int LG>
+__global__ __launch_bounds__(1024) void k_dv_bucket_order(T *__restrict__ d,
+                                                          const unsigned short *__restrict__ off,
+                                                          size_t n) {
+    constexpr int PER = (1 << LG) / 1024;
+    __shared__ T lds[1 << LG];
+    const size_t base = (size_t)blockIdx.x << LG;
+    const int cnt = (int)min((size_t)1 << LG, n - base);
+    unsigned short o[PER];
+    T x[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int t = threadIdx.x + i * 1024;
+        o[i] = t < cnt ? off[base + t] : (unsigned short)0;
+        x[i] = t < cnt ? d[base + t] : (T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++)
+        if ((int)threadIdx.x + i * 1024 < cnt) lds[o[i]] = x[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int t = threadIdx.x + i * 1024;
+        if (t < cnt) d[base + t] = lds[t];
+    }
+}
+
 template <class T>
 __global__ void k_zero_rows_bw(const int *__restrict__ rows, int nrows, T *__restrict__ D, int K) {
     const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
@@ -611,7 +792,7 @@ __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
 // atomics, as the forward launch does, cost more than it saved: the
 // transposed 1M-row matrix has many of them, fused backward 1.06 -> 1.13 ms,
 // and the flag took the 8-lane walk from 6 to 5 waves per SIMD.)
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV>
+template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK>
 static int launch_bw(const BwArgs &a0, hipStream_t s) {
     BwArgs a = a0;
     constexpr bool fuse = !DB;
@@ -620,10 +801,10 @@ static int launch_bw(const BwArgs &a0, hipStream_t s) {
     const dim3 grid((unsigned)(a.walk_blocks + (fuse ? res_blocks : 0)));
     if (grid.x) {
         if (a.gstride)
-            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, MASK, DB, DV, true>), grid, dim3(256), 0,
+            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, MASK, DB, DV, WALK, true>), grid, dim3(256), 0,
                                s, a);
         else
-            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, MASK, DB, DV, false>), grid, dim3(256),
+            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, MASK, DB, DV, WALK, false>), grid, dim3(256),
                                0, s, a);
     }
     if (!fuse && res_blocks)
@@ -639,7 +820,7 @@ static int launch_bw(const BwArgs &a0, hipStream_t s) {
 // up to 64.  dB rows split by a segment boundary are zeroed once over all of
 // K; every slice stores its own columns of dB, and dvals takes the first
 // slice's partial dots, then each later slice's added on.
-template <class T, bool DB, bool DV>
+template <class T, bool DB, bool DV, bool WALK>
 static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
     constexpr int V = V16<T>::N;
     const int K = a0.K;
@@ -657,12 +838,12 @@ static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
         a.dv_add = i > 0 ? 1 : 0;
         if (a.gstride) a.gbytes = (unsigned)((a0.gbytes / sizeof(T) - (size_t)a.kbase) * sizeof(T));
         if (t.mask) {
-            SWIFT_TRY((launch_bw<T, 1, 1, 64, true, DB, DV>(a, s)));
+            SWIFT_TRY((launch_bw<T, 1, 1, 64, true, DB, DV, WALK>(a, s)));
             continue;
         }
 #define SWIFT_BW_CASE(Q_, L_)                                                 \
         if (t.q == Q_ && t.lpe == L_) {                                      \
-            SWIFT_TRY((launch_bw<T, V, Q_, L_, false, DB, DV>(a, s)));       \
+            SWIFT_TRY((launch_bw<T, V, Q_, L_, false, DB, DV, WALK>(a, s)));       \
             continue;                                                        \
         }
         SWIFT_BW_CASE(4, 64)
@@ -724,8 +905,8 @@ int sddmm_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, 
     a.dvals = dvals;
     if (dtype == SWIFT_F64) set_g_buffer<double>(a, A, K);
     else set_g_buffer<float>(a, A, K);
-    return dtype == SWIFT_F64 ? dispatch_bw<double, false, true>(a, s)
-                              : dispatch_bw<float, false, true>(a, s);
+    return dtype == SWIFT_F64 ? dispatch_bw<double, false, true, false>(a, s)
+                              : dispatch_bw<float, false, true, false>(a, s);
 }
 
 // Fused backward walking A^T: H = B (per A^T row = A column), G = dC (per
@@ -742,11 +923,54 @@ int backward_impl(const swift_matrix *AT, int K, swift_dtype dtype, const void *
     a.dB = dB;
     a.dvals = dvals;
     const bool f64 = dtype == SWIFT_F64;
+    const size_t vs = f64 ? sizeof(double) : sizeof(float);
+    // dvals in walk order, then two passes through LDS into CSR order: a
+    // transpose's entries sit far apart in the original CSR order, and one
+    // 4-byte store per entry to scattered lines costs more than the reorder
+    const bool walk = dvals && AT->nnz;
+    if (walk) {
+        if (!AT->dv_dst || a.nblk != 0 || AT->dv_log != dv_bucket_log(dtype) ||
+            AT->dv_clog != dv_chunk_log(dtype)) {
+            set_error("swift_backward_fused: the handle is not a transposed layout "
+                      "(use swift_build_transpose)");
+            return SWIFT_ERR_ARG;
+        }
+        SWIFT_TRY(AT->dv_ws.alloc(vs * (size_t)AT->nnz));
+        a.dvals = AT->dv_ws.ptr;
+    }
     if (f64) set_g_buffer<double>(a, AT, K);
     else set_g_buffer<float>(a, AT, K);
-    if (dB && dvals) return f64 ? dispatch_bw<double, true, true>(a, s) : dispatch_bw<float, true, true>(a, s);
-    if (dB) return f64 ? dispatch_bw<double, true, false>(a, s) : dispatch_bw<float, true, false>(a, s);
-    return f64 ? dispatch_bw<double, false, true>(a, s) : dispatch_bw<float, false, true>(a, s);
+    int rc;
+    if (dB && dvals)
+        rc = f64 ? dispatch_bw<double, true, true, true>(a, s)
+                 : dispatch_bw<float, true, true, true>(a, s);
+    else if (dB)
+        rc = f64 ? dispatch_bw<double, true, false, false>(a, s)
+                 : dispatch_bw<float, true, false, false>(a, s);
+    else
+        rc = f64 ? dispatch_bw<double, false, true, true>(a, s)
+                 : dispatch_bw<float, false, true, true>(a, s);
+    if (rc != SWIFT_OK || !walk) return rc;
+    const size_t n = (size_t)AT->nnz;
+    const unsigned nc = (unsigned)ceil_div(n, (size_t)1 << AT->dv_clog);
+    const unsigned nb = (unsigned)ceil_div(n, (size_t)1 << AT->dv_log);
+    if (f64) {
+        using P = DvPlan<double>;
+        hipLaunchKernelGGL((k_dv_to_buckets<double, P::CHUNK>), dim3(nc), dim3(1024), 0, s,
+                           (const double *)AT->dv_ws.ptr, AT->dv_rank, AT->dv_dst, n,
+                           (double *)dvals);
+        hipLaunchKernelGGL((k_dv_bucket_order<double, P::BUCKET>), dim3(nb), dim3(1024), 0, s,
+                           (double *)dvals, AT->dv_off, n);
+    } else {
+        using P = DvPlan<float>;
+        hipLaunchKernelGGL((k_dv_to_buckets<float, P::CHUNK>), dim3(nc), dim3(1024), 0, s,
+                           (const float *)AT->dv_ws.ptr, AT->dv_rank, AT->dv_dst, n,
+                           (float *)dvals);
+        hipLaunchKernelGGL((k_dv_bucket_order<float, P::BUCKET>), dim3(nb), dim3(1024), 0, s,
+                           (float *)dvals, AT->dv_off, n);
+    }
+    SWIFT_CHECK_LAUNCH();
+    return SWIFT_OK;
 }
 
 }  // namespace swift

@@ -681,7 +681,8 @@ static void free_matrix_arrays(swift_matrix *M) {
                     M->rowsorted, M->rowsort_src, M->csr_rowptr, M->csr_colidx,
                     M->sortval_lp, M->reside_val_lp, M->work, M->rv_ptr, M->rv_col,
                     M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->seg, M->split_rows,
-                    M->rv_row, M->empty_rows, M->resrow_flag, M->reside_rows};
+                    M->rv_row, M->empty_rows, M->resrow_flag, M->reside_rows, M->dv_rank,
+                    M->dv_off, M->dv_dst};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : M->cls) {
@@ -690,6 +691,7 @@ static void free_matrix_arrays(swift_matrix *M) {
         t = swift_matrix::SegTable();
     }
     M->scratch.release();
+    M->dv_ws.release();
 }
 
 template <class T>

@@ -171,6 +171,19 @@ struct swift_matrix {
     void *rv_val = nullptr;      // value (vtype)
     int *rv_src = nullptr;       // CSR position of each entry
     int *rv_row = nullptr;       // output row of each entry
+    // transposed layouts (swift_build_transpose, whose row view holds every
+    // entry): the plan that moves dvals from walk order into the original
+    // matrix's CSR order in two passes through LDS.  Chunks are ranges of
+    // 2^dv_clog walk positions, buckets ranges of 2^dv_log CSR positions, and
+    // bucket order lists the entries by bucket, in walk order inside one.
+    //   dv_rank[e]: rank of walk position e inside its chunk, by (bucket, e)
+    //   dv_dst[p]:  slot in bucket order of the chunk's p-th entry by rank
+    //   dv_off[s]:  CSR position, inside its bucket, of bucket-order slot s
+    // dv_ws holds the nnz walk-order values.
+    unsigned short *dv_rank = nullptr, *dv_off = nullptr;
+    int *dv_dst = nullptr;
+    int dv_log = 0, dv_clog = 0;
+    mutable swift::Buffer dv_ws;
     int *empty_rows = nullptr;   // rows with no regular entry (zeroed before a flat walk)
     int nempty = 0;
     void *rv_val_lp = nullptr;   // BF16/F16 copy (swift_prepare_values)

@@ -199,13 +199,48 @@ def check_backward(swift, oracle, name, K, dt, offset=0):
     M.close()
 
 
+# banded_3000 and banded_mixed_4000 hold more entries than one chunk and one
+# bucket of the dvals reordering (8,192 and 32,768 FP32 / 16,384 FP64 values)
 @pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "equalrows_100x70",
-                                  "unsorted_rows_64x96", "narrow_50x20"])
+                                  "unsorted_rows_64x96", "narrow_50x20", "banded_3000",
+                                  "banded_mixed_4000"])
 @pytest.mark.parametrize("K", [32, 128])
 @pytest.mark.parametrize("dt", ["f32", "f64"])
 def test_backward(swift, oracle, cuda, name, K, dt):
     print("case", name, K, dt, flush=True)
     check_backward(swift, oracle, name, K, dt)
+
+
+@pytest.mark.parametrize("K", [100, 128])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_backward_reorder_many_buckets(swift, oracle, cuda, K, dt):
+    """The benchmark matrix's structure at 1/20 scale (50,000 rows, ~800k
+    entries: ~100 chunks and 25-50 buckets of the dvals reordering, the last
+    ones partial); K = 100 adds later K slices onto the walk-order dvals
+    before the reorder.  Fused dvals and dB against the FP64 oracle."""
+    import torch
+    d = swift.generate(2, scale=0.05, seed=3)
+    m, n, rp, ci = d["m"], d["n"], d["rowptr"], d["colidx"]
+    npdt = np.float32 if dt == "f32" else np.float64
+    tol = 1e-5 if dt == "f32" else 1e-12
+    v = d["val"].astype(npdt)
+    print("case reorder", m, d["nnz"], K, dt, flush=True)
+    rng = np.random.default_rng(K)
+    B = rng.uniform(-1, 1, (n, K)).astype(npdt)
+    dC = rng.uniform(-1, 1, (m, K)).astype(npdt)
+    v64, B64, dC64 = v.astype(np.float64), B.astype(np.float64), dC.astype(np.float64)
+    dv64 = oracle.sddmm_f64(m, n, rp, ci, dC64, B64)
+    absv = oracle.sddmm_f64(m, n, rp, ci, np.abs(dC64), np.abs(B64))
+    dB64 = oracle.spmm_t_f64(m, n, rp, ci, v64, dC64)
+    absB = oracle.spmm_t_f64(m, n, rp, ci, np.abs(v64), np.abs(dC64))
+    M = swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float32 if dt == "f32" else torch.float64)
+    dB, dv = M.backward_fused(torch.tensor(dC, device="cuda"), torch.tensor(B, device="cuda"))
+    got = dv.cpu().numpy().astype(np.float64)
+    err = float(np.max(np.abs(got - dv64) / (absv + 1e-30)))
+    print("dvals err %.3g" % err, flush=True)
+    assert err <= tol
+    assert gpu_common.rel_err(dB.cpu().numpy(), dB64, absB) <= tol
+    M.close()
 
 
 # the K sweep of test/irregular_K (script/Run_irregular_K.sh) plus K = 100 and
