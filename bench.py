@@ -39,9 +39,75 @@ def algorithmic_bytes(m, n, nnz, K, s_a=4, s_b=4, s_c=4):
     return nnz * (4 + s_a) + 4 * (m + 1) + n * K * s_b + m * K * s_c
 
 
+ROCPROF = "/opt/rocm/bin/rocprofv3"
+KERNEL_PREFIX = {"f32": "k_rowview_row<float, float, float,",
+                 "bf16": "k_rowview_row<__hip_bfloat16, __hip_bfloat16, float,"}
+
+
+def under_profiler():
+    return "rocprof" in os.environ.get("LD_PRELOAD", "") or any(
+        k.startswith("ROCPROFILER") for k in os.environ)
+
+
+def live_traffic(storage):
+    """HBM bytes per launch of the dominant kernel, measured in this run: two
+    rocprofv3 --pmc child runs of this script's short step (--calibrate),
+    FETCH_SIZE then WRITE_SIZE in KiB per dispatch, each scaled by what the
+    same counter reads for the 256 MiB device copy that --calibrate runs
+    first.  None when the profiler is missing or a run fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    if not os.access(ROCPROF, os.X_OK):
+        return None
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="swift_pmc_", dir="/tmp")
+        cmd = [ROCPROF, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__), "--steps", "3", "--warmup", "1",
+               "--no-cpu", "--no-vendor", "--no-configs", "--batch", "0", "--calibrate",
+               "--storage", storage]
+        kern, copy = [], []
+        try:
+            # the profiler may end with a signal after the results are written:
+            # the files decide, not the exit code
+            subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                           capture_output=True, timeout=240)
+            rows = []
+            for root, _, files in os.walk(out):
+                for f in files:
+                    if f.endswith("counter_collection.csv"):
+                        with open(os.path.join(root, f)) as fh:
+                            rows += [r for r in csv.DictReader(fh) if r["Counter_Name"] == counter]
+            rows.sort(key=lambda r: int(r.get("Dispatch_Id") or 0))
+            for r in rows:
+                name = r["Kernel_Name"]
+                if "copyBuffer" in name:
+                    copy.append(float(r["Counter_Value"]))
+                elif KERNEL_PREFIX[storage] in name:
+                    kern.append(float(r["Counter_Value"]))
+        except (OSError, subprocess.TimeoutExpired, KeyError, ValueError):
+            kern, copy = [], []
+        finally:
+            shutil.rmtree(out, ignore_errors=True)
+        if not kern or not copy or copy[0] <= 0:
+            log("live %s of the %s walk: no result" % (counter, storage))
+            return None
+        # the calibration copy is the run's first device copy
+        got[counter] = (sum(kern) / len(kern), (256 << 10) / copy[0], len(kern))
+    (fk, ff, nd), (wk, wf, _) = got["FETCH_SIZE"], got["WRITE_SIZE"]
+    traffic = int(round(fk * 1024 * ff + wk * 1024 * wf))
+    return {"traffic": traffic, "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+            "fetch_factor": round(ff, 3), "write_factor": round(wf, 3), "dispatches": nd,
+            "source": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, child runs of "
+                      "bench.py --calibrate --storage %s in this run" % storage}
+
+
 def measured_traffic(path):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy calibration)."""
+    FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy calibration):
+    the fallback when the live measurement is not available."""
     try:
         for line in open(path):
             f = line.split()
@@ -153,7 +219,11 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the configuration table (the other benchmark configurations)")
     ap.add_argument("--calibrate", action="store_true",
-                    help="first run a plain 256 MiB device copy (counter calibration)")
+                    help="first run a plain 256 MiB device copy (counter calibration) and "
+                         "time the step only (the counter runs of the traffic measurement)")
+    ap.add_argument("--no-live-traffic", action="store_true",
+                    help="take roofline.traffic from the committed rocprofv3 summary instead "
+                         "of measuring it with rocprofv3 child runs")
     ap.add_argument("--kernel", default="DEFAULT",
                     help="row-major kernel family (DEFAULT, ROWVIEW, WINDOW, FLAT, BALANCED)")
     ap.add_argument("--part", choices=("all", "banded", "zipf"), default="all",
@@ -281,13 +351,14 @@ def main():
                      "algorithmic_bytes": alg},
         "gpu": torch.cuda.get_device_name(0),
     }
-    if args.part == "all" and not bf16:
+    if args.part == "all" and not bf16 and not args.calibrate:
         t0 = time.time()
         out["forms"] = headline_forms(S, torch, M, d, B, K, ms_step)
         log("forms %.1fs: %s" % (time.time() - t0, out["forms"]))
     if args.extras:
         out["extras"] = extras(S, torch, M, d, B, K)
-    table = args.part == "all" and not bf16 and not args.no_configs and K == 128
+    table = (args.part == "all" and not bf16 and not args.no_configs and K == 128 and
+             not args.calibrate)
     if table:
         t0 = time.time()
         out["configs"] = config_table(S, torch, M, d, B, ms_step)
@@ -315,6 +386,22 @@ def main():
         out["cpu_baseline"] = cpu_baseline(d, K, seed)
         out["cpu_baseline"].update(host_cpu())
         log("cpu baseline %.1fs: %s" % (time.time() - t0, out["cpu_baseline"]))
+    # HBM traffic of the dominant kernel, measured now (after every timed part)
+    if default and not args.calibrate and not args.no_live_traffic and not under_profiler():
+        forms = [("f32", out["roofline"], reg_ms)] if not bf16 else [("bf16", out["roofline"], reg_ms)]
+        if "forms" in out:
+            forms.append(("bf16", out["forms"]["bf16"]["roofline"], out["forms"]["bf16"]["regular_ms"]))
+        for storage, roof, ms in forms:
+            t0 = time.time()
+            lt = live_traffic(storage)
+            log("live traffic %s %.1fs: %s" % (storage, time.time() - t0, lt))
+            if lt:
+                roof["traffic"] = lt.pop("traffic")
+                roof["traffic_frac"] = round(roof["traffic"] / (ms * 1e-3) / HBM_PEAK, 4)
+                roof["traffic_measurement"] = lt
+            elif roof.get("traffic"):
+                roof["traffic_measurement"] = {"source": "committed rocprofv3 summary under "
+                                                         "profiles/ (live run unavailable)"}
     print(json.dumps(out), flush=True)
 
 
