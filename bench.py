@@ -46,7 +46,7 @@ KERNEL_PREFIX = {"f32": "k_rowview_row<float, float, float,",
 
 def under_profiler():
     return "rocprof" in os.environ.get("LD_PRELOAD", "") or any(
-        k.startswith("ROCPROFILER") for k in os.environ)
+        k.startswith("ROCPROF") for k in os.environ)
 
 
 def live_traffic(storage):
