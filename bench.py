@@ -538,11 +538,13 @@ def config_table(S, torch, M, d, B, f32_ms):
     return res
 
 
-def backward_bytes(m, n, nnz, K, s=4):
-    """Fused backward over A^T: per entry its column, value, CSR source index
-    and dvals once (4 + s + 4 + s bytes), A^T's row pointer, dC gathered once
-    (m*K), B and dB once each (n*K)."""
-    return nnz * (8 + 2 * s) + 4 * (n + 1) + m * K * s + 2 * n * K * s
+def backward_bytes(m, n, nnz, K, s=4, s_out=None):
+    """Fused backward over A^T: per entry its column, value, CSR position and
+    dvals once (4 + s + 4 + s_out bytes), A^T's row pointer, dC gathered once
+    (m*K), B and dB once each (n*K); s is the stored element size of dC, B and
+    the values, s_out that of dB and dvals (= s unless BF16 storage)."""
+    s_out = s if s_out is None else s_out
+    return nnz * (8 + s + s_out) + 4 * (n + 1) + (m + n) * K * s + n * K * s_out
 
 
 def gnn_config(S, torch, scale):
@@ -564,6 +566,10 @@ def gnn_config(S, torch, scale):
     M.prepare_values(S.BF16)
     Bh = S.convert_dense(B, S.BF16)
     fbf = time_call(torch, lambda: M.spmm(Bh, out=C, dtype=S.BF16), reps=20)
+    dCh = S.convert_dense(dC, S.BF16)
+    M.backward_fused(dCh, Bh)  # prepares the transposed layout's BF16 values once
+    bbf = time_call(torch, lambda: M.backward_fused(dCh, Bh), reps=20)
+    del dCh
     # last: the autotuner pins the matrix's segment size for every later product
     M.autotune(B, C)
     inf = M.info()
@@ -575,6 +581,10 @@ def gnn_config(S, torch, scale):
                                        algorithmic_bytes(m, n, nnz, K, s_a=2, s_b=2)),
         "C4 graph fused backward": entry(bwd, 4.0 * nnz * K, backward_bytes(m, n, nnz, K)),
         "C4 graph fwd+bwd / fwd": round((fwd + bwd) / fwd, 3),
+        "C4 graph bf16 fused backward": entry(bbf, 4.0 * nnz * K,
+                                              backward_bytes(m, n, nnz, K, s=2, s_out=4)),
+        "C4 graph bf16 fwd+bwd / fwd": round((fbf + bbf) / fbf, 3),
+        "C4 graph bf16 fwd+bwd speedup vs f32": round((fwd + bwd) / (fbf + bbf), 3),
         "C4 graph f32 forward autotuned": dict(
             entry(ftu, 2.0 * nnz * K, algorithmic_bytes(m, n, nnz, K)), default_ms=round(fwd, 4),
             schedule={"segment_items": inf.segment_items, "slots": inf.tuned_slots,

@@ -341,7 +341,10 @@ int swift_sddmm(const swift_matrix *A, int K, swift_dtype dtype,
 /* Fused backward in one pass over the transposed Swift layout AT (from
  * swift_build_transpose): dB = A^T*dC (n x K) and dvals = SDDMM(dC, B) in
  * A's CSR order; every dC row is gathered once per nonzero and feeds both.
- * Either output may be NULL.  Row-major, original column order, F32/F64.
+ * Either output may be NULL.  Row-major, original column order, F32/F64;
+ * or dtype BF16 for a transpose of an FP32 build: dC, B and AT's values
+ * (swift_prepare_values(AT, BF16) first) in BF16, with FP32 products, sums,
+ * dB and dvals.
  * dvals are written in AT's walk order into a workspace of nnz values held
  * by AT, then gathered into CSR order: calls with dvals on one AT must not
  * overlap in time (issue them on one stream). */

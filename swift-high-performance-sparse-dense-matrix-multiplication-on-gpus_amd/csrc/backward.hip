@@ -282,9 +282,12 @@ struct BwArgs {
     int walk_blocks;
 };
 
+// Elements per lane load of a stored operand: 16 bytes for FP32 / FP64, 8
+// bytes (four values) for BF16 storage, as in the forward walk.
 template <class T> struct V16;
 template <> struct V16<float> { static constexpr int N = 4; };
 template <> struct V16<double> { static constexpr int N = 2; };
+template <> struct V16<__hip_bfloat16> { static constexpr int N = 4; };
 
 template <class T, int VEC>
 __device__ __forceinline__ void ldv(const T *p, T (&o)[VEC]) {
@@ -313,6 +316,18 @@ __device__ __forceinline__ void ldv_stream(const T *p, T (&o)[VEC]) {
 #pragma unroll
         for (int i = 0; i < VEC; i++) o[i] = ld_stream(p + i);
     }
+}
+
+// VEC elements of a stored operand (TS) widened to the compute type T.
+template <class TS, class T, int VEC>
+__device__ __forceinline__ void ldw(const TS *p, T (&o)[VEC]) {
+    if constexpr (std::is_same<TS, T>::value) ldv<T, VEC>(p, o);
+    else load_b<TS, VEC, T>(p, o);
+}
+template <class TS, class T, int VEC>
+__device__ __forceinline__ void ldw_stream(const TS *p, T (&o)[VEC]) {
+    if constexpr (std::is_same<TS, T>::value) ldv_stream<T, VEC>(p, o);
+    else load_b<TS, VEC, T>(p, o);
 }
 
 // dB rows are written once: non-temporal stores.
@@ -385,7 +400,7 @@ __device__ __forceinline__ void put_dval(const BwArgs &a, int slot, T r) {
 // DB: acc += v * G[c] for the row's dB.  DV: dvals[src] = <G[c], H[row]>, the
 // slots' partial dots summed over the lane group together (lane sub then
 // holds slot sub / (LPE / UNROLL)).
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK, int UNROLL, int N>
+template <class TS, class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK, int UNROLL, int N>
 __device__ __forceinline__ void bw_body(const BwArgs &a, __amdgpu_buffer_rsrc_t rsrc, int base,
                                         int j, int nb, int h, int sub, int mc, int ms, T mv,
                                         const T (&hr)[Q][VEC], T (&acc)[Q][VEC], int pos) {
@@ -407,7 +422,7 @@ __device__ __forceinline__ void bw_body(const BwArgs &a, __amdgpu_buffer_rsrc_t 
         for (int q = 0; q < Q; q++) {
             const int k0 = q * LPE * VEC + sub * VEC;
             const bool in = ok && (!MASK || k0 < a.kcount);
-            load_b_buf<T, VEC, T>(rsrc, in ? roff + (unsigned)(k0 * (int)sizeof(T)) : OOB_OFFSET,
+            load_b_buf<TS, VEC, T>(rsrc, in ? roff + (unsigned)(k0 * (int)sizeof(TS)) : OOB_OFFSET,
                                   g[u][q]);
         }
     }
@@ -443,26 +458,26 @@ __device__ __forceinline__ void bw_body(const BwArgs &a, __amdgpu_buffer_rsrc_t 
 }
 
 // The body for nu = min(UNROLL, slot rows left) slot rows (wave-uniform).
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK, int UNROLL, int N>
+template <class TS, class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK, int UNROLL, int N>
 __device__ __forceinline__ void bw_pick(int nu, const BwArgs &a, __amdgpu_buffer_rsrc_t rsrc,
                                         int base, int j, int nb, int h, int sub, int mc, int ms,
                                         T mv, const T (&hr)[Q][VEC], T (&acc)[Q][VEC], int pos) {
     if constexpr (N == 1) {
-        bw_body<T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, 1>(a, rsrc, base, j, nb, h, sub, mc, ms, mv,
+        bw_body<TS, T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, 1>(a, rsrc, base, j, nb, h, sub, mc, ms, mv,
                                                          hr, acc, pos);
     } else {
         if (nu >= N)
-            bw_body<T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, N>(a, rsrc, base, j, nb, h, sub, mc, ms,
+            bw_body<TS, T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, N>(a, rsrc, base, j, nb, h, sub, mc, ms,
                                                              mv, hr, acc, pos);
         else
-            bw_pick<T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, N - 1>(nu, a, rsrc, base, j, nb, h, sub,
+            bw_pick<TS, T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, N - 1>(nu, a, rsrc, base, j, nb, h, sub,
                                                                  mc, ms, mv, hr, acc, pos);
     }
 }
 
 // Reside blocks of the walked matrix: column operand G[c] loaded once, each
 // entry gathers H[row]; dvals = <H[row], G[c]>, dB[row] += v * G[c].
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV>
+template <class TS, class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV>
 __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
     constexpr int EPS = 64 / LPE;
     const int lane = threadIdx.x & 63;
@@ -473,9 +488,9 @@ __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
     const int start = split ? a.blkstart[blk] : a.rptr[r];
     const int stop = split ? a.blkstop[blk] : a.rptr[r + 1];
     const int c = a.perm[a.n - 1 - r];
-    const T *G = static_cast<const T *>(a.G) + a.kbase;
-    const T *H = static_cast<const T *>(a.H) + a.kbase;
-    const T *rval = static_cast<const T *>(a.rval);
+    const TS *G = static_cast<const TS *>(a.G) + a.kbase;
+    const TS *H = static_cast<const TS *>(a.H) + a.kbase;
+    const TS *rval = static_cast<const TS *>(a.rval);
     // lane-strided layout k = q*LPE*VEC + x*LPE + sub: for a fixed (q, x) the
     // LPE lanes touch LPE consecutive elements, so each dB atomic instruction
     // covers whole cache lines
@@ -485,7 +500,7 @@ __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
 #pragma unroll
         for (int x = 0; x < VEC; x++) {
             const int k = q * LPE * VEC + x * LPE + sub;
-            gc[q][x] = (!MASK || k < a.kcount) ? G[(size_t)c * a.K + k] : (T)0;
+            gc[q][x] = (!MASK || k < a.kcount) ? to_acc(G[(size_t)c * a.K + k]) : (T)0;
         }
     for (int e0 = start; e0 < stop; e0 += EPS) {
         const int e = e0 + h;
@@ -498,13 +513,13 @@ __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
 #pragma unroll
                 for (int x = 0; x < VEC; x++) {
                     const int k = q * LPE * VEC + x * LPE + sub;
-                    if (ok && (!MASK || k < a.kcount)) d += H[(size_t)row * a.K + k] * gc[q][x];
+                    if (ok && (!MASK || k < a.kcount)) d += to_acc(H[(size_t)row * a.K + k]) * gc[q][x];
                 }
             d = group_sum<T, LPE>(d);
             if (ok && sub == 0) put_dval(a, a.rsrc[e], d);
         }
         if (DB && ok) {
-            const T v = rval[e];
+            const T v = to_acc(rval[e]);
             T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K + a.kbase;
 #pragma unroll
             for (int q = 0; q < Q; q++)
@@ -523,13 +538,13 @@ __device__ __forceinline__ void bw_reside_block(const BwArgs &a, int blk) {
 #ifndef SWIFT_BW_UNROLL_LONG
 #define SWIFT_BW_UNROLL_LONG 4
 #endif
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK, bool BUF>
+template <class TS, class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK, bool BUF>
 __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     if constexpr (!DB) {
         if (blockIdx.x >= (unsigned)a.walk_blocks) {
             const int blk =
                 (int)(((blockIdx.x - a.walk_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
-            if (blk < a.nblk) bw_reside_block<T, VEC, Q, LPE, MASK, DB, DV>(a, blk);
+            if (blk < a.nblk) bw_reside_block<TS, T, VEC, Q, LPE, MASK, DB, DV>(a, blk);
             return;
         }
     }
@@ -545,9 +560,9 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     const int r1 = en.x, e1 = en.y;
     int row = st.x, e = st.y;
     const bool carried_in = row < a.m && e > a.rv_ptr[row];
-    const T *val = static_cast<const T *>(a.rv_val);
-    const T *G = static_cast<const T *>(a.G) + a.kbase;
-    const T *H = static_cast<const T *>(a.H) + a.kbase;
+    const TS *val = static_cast<const TS *>(a.rv_val);
+    const TS *G = static_cast<const TS *>(a.G) + a.kbase;
+    const TS *H = static_cast<const TS *>(a.H) + a.kbase;
     __amdgpu_buffer_rsrc_t rsrc;
     if constexpr (BUF) rsrc = make_rsrc(G, a.gbytes);
     // staged row ends and entries, as in the forward row-view kernel
@@ -558,7 +573,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     if (ebase + lane < e1) {
         mc = ld_stream(a.rv_col + ebase + lane);
         if (DV && !WALK) ms = ld_stream(a.rv_src + ebase + lane);
-        mv = ld_stream(val + ebase + lane);
+        mv = to_acc(ld_stream(val + ebase + lane));
     }
     T acc[Q][VEC], hr[Q][VEC];
 #pragma unroll
@@ -576,7 +591,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const int k0 = q * LPE * VEC + sub * VEC;
-                if (!MASK || k0 < a.kcount) ldv_stream<T, VEC>(H + (size_t)row * a.K + k0, hr[q]);
+                if (!MASK || k0 < a.kcount) ldw_stream<TS, T, VEC>(H + (size_t)row * a.K + k0, hr[q]);
             }
         }
         while (e < row_end) {
@@ -587,7 +602,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 if (ebase + lane < e1) {
                     mc = ld_stream(a.rv_col + ebase + lane);
                     if (DV && !WALK) ms = ld_stream(a.rv_src + ebase + lane);
-                    mv = ld_stream(val + ebase + lane);
+                    mv = to_acc(ld_stream(val + ebase + lane));
                 }
             }
             const int off = e - ebase;
@@ -598,7 +613,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 int src[UNROLL];
                 if constexpr (BUF) {
                     const int nu = min(UNROLL, (nb - j + EPS - 1) / EPS);
-                    bw_pick<T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, UNROLL>(
+                    bw_pick<TS, T, VEC, Q, LPE, MASK, DB, DV, WALK, UNROLL, UNROLL>(
                         nu, a, rsrc, (off + j + h) << 2, j, nb, h, sub, mc, ms, mv, hr, acc,
                         e + j + h);
                     continue;
@@ -616,7 +631,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                     for (int q = 0; q < Q; q++) {
                         const int k0 = q * LPE * VEC + sub * VEC;
                         if (idx < nb && (!MASK || k0 < a.kcount))
-                            ldv<T, VEC>(G + (size_t)c * a.K + k0, g[u][q]);
+                            ldw<TS, T, VEC>(G + (size_t)c * a.K + k0, g[u][q]);
                         else
 #pragma unroll
                             for (int v = 0; v < VEC; v++) g[u][q][v] = (T)0;
@@ -779,10 +794,10 @@ __global__ void k_zero_rows_bw(const int *__restrict__ rows, int nrows, T *__res
     for (int k = threadIdx.x & 63; k < K; k += 64) d[k] = (T)0;
 }
 
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV>
+template <class TS, class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV>
 __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
     const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (blk < a.nblk) bw_reside_block<T, VEC, Q, LPE, MASK, DB, DV>(a, blk);
+    if (blk < a.nblk) bw_reside_block<TS, T, VEC, Q, LPE, MASK, DB, DV>(a, blk);
 }
 
 // dvals alone (SDDMM): the reside blocks run as extra blocks of the walk's
@@ -792,7 +807,7 @@ __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
 // atomics, as the forward launch does, cost more than it saved: the
 // transposed 1M-row matrix has many of them, fused backward 1.06 -> 1.13 ms,
 // and the flag took the 8-lane walk from 6 to 5 waves per SIMD.)
-template <class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK>
+template <class TS, class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV, bool WALK>
 static int launch_bw(const BwArgs &a0, hipStream_t s) {
     BwArgs a = a0;
     constexpr bool fuse = !DB;
@@ -801,14 +816,14 @@ static int launch_bw(const BwArgs &a0, hipStream_t s) {
     const dim3 grid((unsigned)(a.walk_blocks + (fuse ? res_blocks : 0)));
     if (grid.x) {
         if (a.gstride)
-            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, MASK, DB, DV, WALK, true>), grid, dim3(256), 0,
+            hipLaunchKernelGGL((k_bw_walk<TS, T, VEC, Q, LPE, MASK, DB, DV, WALK, true>), grid, dim3(256), 0,
                                s, a);
         else
-            hipLaunchKernelGGL((k_bw_walk<T, VEC, Q, LPE, MASK, DB, DV, WALK, false>), grid, dim3(256),
+            hipLaunchKernelGGL((k_bw_walk<TS, T, VEC, Q, LPE, MASK, DB, DV, WALK, false>), grid, dim3(256),
                                0, s, a);
     }
     if (!fuse && res_blocks)
-        hipLaunchKernelGGL((k_bw_reside<T, VEC, Q, LPE, MASK, DB, DV>), dim3(res_blocks),
+        hipLaunchKernelGGL((k_bw_reside<TS, T, VEC, Q, LPE, MASK, DB, DV>), dim3(res_blocks),
                            dim3(256), 0, s, a);
     SWIFT_CHECK_LAUNCH();
     return SWIFT_OK;
@@ -820,9 +835,9 @@ static int launch_bw(const BwArgs &a0, hipStream_t s) {
 // up to 64.  dB rows split by a segment boundary are zeroed once over all of
 // K; every slice stores its own columns of dB, and dvals takes the first
 // slice's partial dots, then each later slice's added on.
-template <class T, bool DB, bool DV, bool WALK>
+template <class TS, class T, bool DB, bool DV, bool WALK>
 static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
-    constexpr int V = V16<T>::N;
+    constexpr int V = V16<TS>::N;
     const int K = a0.K;
     BwArgs a = a0;
     const bool aligned = K % V == 0 &&
@@ -836,14 +851,14 @@ static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
         a.kbase = slices[i].first;
         a.kcount = t.mask ? std::min(64, K - a.kbase) : t.vec * t.lpe * t.q;
         a.dv_add = i > 0 ? 1 : 0;
-        if (a.gstride) a.gbytes = (unsigned)((a0.gbytes / sizeof(T) - (size_t)a.kbase) * sizeof(T));
+        if (a.gstride) a.gbytes = (unsigned)((a0.gbytes / sizeof(TS) - (size_t)a.kbase) * sizeof(TS));
         if (t.mask) {
-            SWIFT_TRY((launch_bw<T, 1, 1, 64, true, DB, DV, WALK>(a, s)));
+            SWIFT_TRY((launch_bw<TS, T, 1, 1, 64, true, DB, DV, WALK>(a, s)));
             continue;
         }
 #define SWIFT_BW_CASE(Q_, L_)                                                 \
         if (t.q == Q_ && t.lpe == L_) {                                      \
-            SWIFT_TRY((launch_bw<T, V, Q_, L_, false, DB, DV, WALK>(a, s)));       \
+            SWIFT_TRY((launch_bw<TS, T, V, Q_, L_, false, DB, DV, WALK>(a, s)));       \
             continue;                                                        \
         }
         SWIFT_BW_CASE(4, 64)
@@ -905,72 +920,72 @@ int sddmm_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, 
     a.dvals = dvals;
     if (dtype == SWIFT_F64) set_g_buffer<double>(a, A, K);
     else set_g_buffer<float>(a, A, K);
-    return dtype == SWIFT_F64 ? dispatch_bw<double, false, true, false>(a, s)
-                              : dispatch_bw<float, false, true, false>(a, s);
+    return dtype == SWIFT_F64 ? dispatch_bw<double, double, false, true, false>(a, s)
+                              : dispatch_bw<float, float, false, true, false>(a, s);
 }
 
 // Fused backward walking A^T: H = B (per A^T row = A column), G = dC (per
-// entry, gathered once), dB rows are A^T's rows.
-int backward_impl(const swift_matrix *AT, int K, swift_dtype dtype, const void *dC, const void *B,
-                  void *dB, void *dvals, hipStream_t s) {
-    if (!AT || K <= 0 || !dC || (!dB && !dvals) || (dvals && !B) || dtype != AT->vtype) {
-        set_error("swift_backward_fused: bad arguments (dtype must equal the build value type)");
-        return SWIFT_ERR_ARG;
-    }
+// entry, gathered once), dB rows are A^T's rows.  dtype BF16: dC, B and A^T's
+// values (swift_prepare_values on AT) in BF16, FP32 products, sums, dB and
+// dvals; AT must be built from an FP32 matrix.
+template <class TS, class T>
+static int backward_t(const swift_matrix *AT, int K, const void *dC, const void *B, void *dB,
+                      void *dvals, hipStream_t s) {
     BwArgs a = walk_args(AT, K);
+    if (!std::is_same<TS, T>::value) a.rv_val = AT->rv_val_lp;
     a.H = B;
     a.G = dC;
     a.dB = dB;
     a.dvals = dvals;
-    const bool f64 = dtype == SWIFT_F64;
-    const size_t vs = f64 ? sizeof(double) : sizeof(float);
     // dvals in walk order, then two passes through LDS into CSR order: a
     // transpose's entries sit far apart in the original CSR order, and one
     // 4-byte store per entry to scattered lines costs more than the reorder
     const bool walk = dvals && AT->nnz;
     if (walk) {
-        if (!AT->dv_dst || a.nblk != 0 || AT->dv_log != dv_bucket_log(dtype) ||
-            AT->dv_clog != dv_chunk_log(dtype)) {
+        if (!AT->dv_dst || a.nblk != 0 || AT->dv_log != DvPlan<T>::BUCKET ||
+            AT->dv_clog != DvPlan<T>::CHUNK) {
             set_error("swift_backward_fused: the handle is not a transposed layout "
                       "(use swift_build_transpose)");
             return SWIFT_ERR_ARG;
         }
-        SWIFT_TRY(AT->dv_ws.alloc(vs * (size_t)AT->nnz));
+        SWIFT_TRY(AT->dv_ws.alloc(sizeof(T) * (size_t)AT->nnz));
         a.dvals = AT->dv_ws.ptr;
     }
-    if (f64) set_g_buffer<double>(a, AT, K);
-    else set_g_buffer<float>(a, AT, K);
+    set_g_buffer<TS>(a, AT, K);
     int rc;
-    if (dB && dvals)
-        rc = f64 ? dispatch_bw<double, true, true, true>(a, s)
-                 : dispatch_bw<float, true, true, true>(a, s);
-    else if (dB)
-        rc = f64 ? dispatch_bw<double, true, false, false>(a, s)
-                 : dispatch_bw<float, true, false, false>(a, s);
-    else
-        rc = f64 ? dispatch_bw<double, false, true, true>(a, s)
-                 : dispatch_bw<float, false, true, true>(a, s);
+    if (dB && dvals) rc = dispatch_bw<TS, T, true, true, true>(a, s);
+    else if (dB) rc = dispatch_bw<TS, T, true, false, false>(a, s);
+    else rc = dispatch_bw<TS, T, false, true, true>(a, s);
     if (rc != SWIFT_OK || !walk) return rc;
     const size_t n = (size_t)AT->nnz;
-    const unsigned nc = (unsigned)ceil_div(n, (size_t)1 << AT->dv_clog);
-    const unsigned nb = (unsigned)ceil_div(n, (size_t)1 << AT->dv_log);
-    if (f64) {
-        using P = DvPlan<double>;
-        hipLaunchKernelGGL((k_dv_to_buckets<double, P::CHUNK>), dim3(nc), dim3(1024), 0, s,
-                           (const double *)AT->dv_ws.ptr, AT->dv_rank, AT->dv_dst, n,
-                           (double *)dvals);
-        hipLaunchKernelGGL((k_dv_bucket_order<double, P::BUCKET>), dim3(nb), dim3(1024), 0, s,
-                           (double *)dvals, AT->dv_off, n);
-    } else {
-        using P = DvPlan<float>;
-        hipLaunchKernelGGL((k_dv_to_buckets<float, P::CHUNK>), dim3(nc), dim3(1024), 0, s,
-                           (const float *)AT->dv_ws.ptr, AT->dv_rank, AT->dv_dst, n,
-                           (float *)dvals);
-        hipLaunchKernelGGL((k_dv_bucket_order<float, P::BUCKET>), dim3(nb), dim3(1024), 0, s,
-                           (float *)dvals, AT->dv_off, n);
-    }
+    using P = DvPlan<T>;
+    hipLaunchKernelGGL((k_dv_to_buckets<T, P::CHUNK>), dim3(ceil_div(n, (size_t)1 << P::CHUNK)),
+                       dim3(1024), 0, s, (const T *)AT->dv_ws.ptr, AT->dv_rank, AT->dv_dst, n,
+                       (T *)dvals);
+    hipLaunchKernelGGL((k_dv_bucket_order<T, P::BUCKET>), dim3(ceil_div(n, (size_t)1 << P::BUCKET)),
+                       dim3(1024), 0, s, (T *)dvals, AT->dv_off, n);
     SWIFT_CHECK_LAUNCH();
     return SWIFT_OK;
+}
+
+int backward_impl(const swift_matrix *AT, int K, swift_dtype dtype, const void *dC, const void *B,
+                  void *dB, void *dvals, hipStream_t s) {
+    const bool bf16 = AT && dtype == SWIFT_BF16 && AT->vtype == SWIFT_F32;
+    if (!AT || K <= 0 || !dC || (!dB && !dvals) || (dvals && !B) ||
+        (dtype != AT->vtype && !bf16)) {
+        set_error("swift_backward_fused: bad arguments (dtype must equal the build value type, "
+                  "or be BF16 for an FP32 build)");
+        return SWIFT_ERR_ARG;
+    }
+    if (bf16) {
+        if (!AT->rv_val_lp || AT->lp_type != SWIFT_BF16) {
+            set_error("swift_backward_fused: BF16 needs swift_prepare_values(AT, BF16) first");
+            return SWIFT_ERR_ARG;
+        }
+        return backward_t<__hip_bfloat16, float>(AT, K, dC, B, dB, dvals, s);
+    }
+    return dtype == SWIFT_F64 ? backward_t<double, double>(AT, K, dC, B, dB, dvals, s)
+                              : backward_t<float, float>(AT, K, dC, B, dB, dvals, s);
 }
 
 }  // namespace swift

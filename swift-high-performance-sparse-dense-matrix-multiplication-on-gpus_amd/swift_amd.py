@@ -430,11 +430,16 @@ class SwiftMatrix:
         return dvals
 
     def backward_fused(self, dC, B, want_dB=True, want_dvals=True):
+        """dB = A^T dC and dvals = SDDMM(dC, B).  BF16 dC and B (FP32 build):
+        BF16 values of A^T (prepared on first use), FP32 dB and dvals."""
         torch = _torch()
         K = dC.shape[1]
-        dB = torch.empty((self.n, K), dtype=dC.dtype, device=dC.device) if want_dB else None
-        dv = torch.empty(self.nnz, dtype=dC.dtype, device=dC.device) if want_dvals else None
+        odt = torch.float32 if dC.dtype == torch.bfloat16 else dC.dtype
+        dB = torch.empty((self.n, K), dtype=odt, device=dC.device) if want_dB else None
+        dv = torch.empty(self.nnz, dtype=odt, device=dC.device) if want_dvals else None
         T = self.transpose()
+        if dC.dtype == torch.bfloat16 and getattr(T, "lowp", None) != BF16:
+            T.prepare_values(BF16)
         check("swift_backward_fused", lib().swift_backward_fused(
             T.h, K, _dtype_code(dC.dtype), _ptr(dC.contiguous()),
             _ptr(B.contiguous()) if B is not None else None, _ptr(dB), _ptr(dv), _stream()))

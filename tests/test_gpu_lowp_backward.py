@@ -254,6 +254,44 @@ def test_backward_irregular_k(swift, oracle, cuda, name, K, dt):
     check_backward(swift, oracle, name, K, dt)
 
 
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "tall_2000x64",
+                                  "banded_mixed_4000"])
+@pytest.mark.parametrize("K", [4, 32, 100, 128, 256])
+def test_backward_bf16(swift, oracle, cuda, name, K):
+    """BF16 storage of dC, B and A's values, FP32 accumulation: a product of
+    two BF16 values is exact in FP32, so against the FP64 oracle on the
+    BF16-rounded operands the error is that of FP32 sums (<= 1e-5 of the
+    absolute sums); against the unrounded FP32 operands it is within the
+    16-bit storage bound (<= 1e-2)."""
+    import torch
+    print("case bf16", name, K, flush=True)
+    m, n, rp, ci, v = CASES[name]
+    rng = np.random.default_rng(K + 7)
+    vv = np.asarray(v, np.float32)
+    B = rng.standard_normal((n, K)).astype(np.float32)
+    dC = rng.standard_normal((m, K)).astype(np.float32)
+    Bh = torch.tensor(B, device="cuda").to(torch.bfloat16)
+    dCh = torch.tensor(dC, device="cuda").to(torch.bfloat16)
+    rnd = lambda x: torch.tensor(x).to(torch.bfloat16).double().numpy()
+    B_r, dC_r, v_r = rnd(B), rnd(dC), rnd(vv)
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=torch.float32)
+    dB, dv = M.backward_fused(dCh, Bh)
+    assert dB.dtype == torch.float32 and dv.dtype == torch.float32
+    dB, dv = dB.cpu().numpy().astype(np.float64), dv.cpu().numpy().astype(np.float64)
+    for (vals, bb, dd), tol in (((v_r, B_r, dC_r), 1e-5),
+                                ((vv.astype(np.float64), B.astype(np.float64),
+                                  dC.astype(np.float64)), 1e-2)):
+        dB64 = oracle.spmm_t_f64(m, n, rp, ci, vals, dd)
+        absB = oracle.spmm_t_f64(m, n, rp, ci, np.abs(vals), np.abs(dd))
+        dv64 = oracle.sddmm_f64(m, n, rp, ci, dd, bb)
+        absv = oracle.sddmm_f64(m, n, rp, ci, np.abs(dd), np.abs(bb))
+        e1 = gpu_common.rel_err(dB, dB64, absB)
+        e2 = float(np.max(np.abs(dv - dv64) / (absv + 1e-30), initial=0))
+        print("tol %g: dB %.3g dvals %.3g" % (tol, e1, e2), flush=True)
+        assert e1 <= tol and e2 <= tol
+    M.close()
+
+
 @pytest.mark.parametrize("K", [32, 100])
 @pytest.mark.parametrize("dt", ["f32", "f64"])
 def test_backward_unaligned_operands(swift, oracle, cuda, K, dt):
