@@ -535,11 +535,19 @@ def spmm_function():
     torch = _torch()
 
     class SwiftSpMM(torch.autograd.Function):
+        """BF16 B (FP32 matrix): BF16 storage of A's values and B with FP32
+        C; the fused backward takes dC rounded to BF16 and returns dB rounded
+        to BF16 and FP32 dvals."""
+
         @staticmethod
         def forward(ctx, vals, B, mat, fused=True):
             mat.update_values(vals.detach())
             ctx.mat, ctx.fused = mat, fused
             ctx.save_for_backward(B)
+            if B.dtype == torch.bfloat16:
+                if mat.lowp != BF16:
+                    mat.prepare_values(BF16)
+                return mat.spmm(B.detach(), dtype=BF16)
             return mat.spmm(B.detach())
 
         @staticmethod
@@ -549,6 +557,10 @@ def spmm_function():
             dC = dC.contiguous()
             need_v, need_b = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
             dB = dv = None
+            if B.dtype == torch.bfloat16:
+                dB, dv = mat.backward_fused(dC.to(torch.bfloat16), B.detach(),
+                                            want_dB=need_b, want_dvals=need_v)
+                return dv, (dB.to(torch.bfloat16) if dB is not None else None), None, None
             if ctx.fused:
                 dB, dv = mat.backward_fused(dC, B.detach(), want_dB=need_b, want_dvals=need_v)
             else:

@@ -374,6 +374,43 @@ def test_autograd_finite_differences(swift, cuda, fused, K):
     M.close()
 
 
+@pytest.mark.parametrize("K", [32, 128])
+def test_autograd_bf16(swift, oracle, cuda, K):
+    """The autograd wrapper with BF16 B (FP32 values): C from BF16 storage,
+    gradients from the BF16 fused backward.  Against FP64 closed forms of
+    loss = <W, A(vals) B>: d/dvals[e] = <W[row_e], B[col_e]> and d/dB = A^T W,
+    with dC = W rounded to BF16 as the backward takes it: within 1e-2 of the
+    absolute sums (16-bit storage of A's values, B and dC)."""
+    import torch
+    d = swift.generate(4, scale=0.01, seed=1)  # 2,000 nodes
+    m, n, rp, ci, v = d["m"], d["n"], d["rowptr"], d["colidx"], d["val"].astype(np.float32)
+    print("case autograd bf16", K, flush=True)
+    M = swift.SwiftMatrix(m, n, rp, ci, v, dtype=torch.float32)
+    Fn = swift.spmm_function()
+    rng = np.random.default_rng(K)
+    vals = torch.tensor(v, device="cuda", requires_grad=True)
+    B = torch.tensor(rng.standard_normal((n, K)), device="cuda",
+                     dtype=torch.float32).to(torch.bfloat16).requires_grad_(True)
+    W = torch.tensor(rng.standard_normal((m, K)), device="cuda", dtype=torch.float32)
+    C = Fn.apply(vals, B, M, True)
+    assert C.dtype == torch.float32
+    (C * W).sum().backward()
+    assert B.grad.dtype == torch.bfloat16 and vals.grad.dtype == torch.float32
+    v64, B64, W64 = v.astype(np.float64), B.detach().double().cpu().numpy(), \
+        W.double().cpu().numpy()
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, v64, B64)
+    assert gpu_common.rel_err(C.detach().cpu().numpy(), C64, absC) <= 1e-2
+    gv64 = oracle.sddmm_f64(m, n, rp, ci, W64, B64)
+    absv = oracle.sddmm_f64(m, n, rp, ci, np.abs(W64), np.abs(B64))
+    gB64 = oracle.spmm_t_f64(m, n, rp, ci, v64, W64)
+    absB = oracle.spmm_t_f64(m, n, rp, ci, np.abs(v64), np.abs(W64))
+    ev = float(np.max(np.abs(vals.grad.double().cpu().numpy() - gv64) / (absv + 1e-30)))
+    eB = gpu_common.rel_err(B.grad.double().cpu().numpy(), gB64, absB)
+    print("dvals err %.3g dB err %.3g" % (ev, eB), flush=True)
+    assert ev <= 1e-2 and eB <= 1e-2
+    M.close()
+
+
 def _gnn_graph(swift):
     d = swift.generate(4, scale=1.0, seed=0)  # 200,000 nodes, ~10.2M edges
     return d["m"], d["rowptr"].astype(np.int64), d["colidx"].astype(np.int64), d["val"]
