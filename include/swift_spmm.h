@@ -334,7 +334,8 @@ int swift_spmm_scaled(const swift_matrix *mat, int K, swift_layout layout,
 int swift_build_transpose(const swift_matrix *A, swift_matrix **AT,
                           swift_stream_t stream);
 /* dvals[e] = <dC[row_e,:], B[col_e,:]> on A's pattern, e in A's CSR order
- * (SDDMM).  Row-major dC (m x K) and B (n x K), original column order. */
+ * (SDDMM).  Row-major dC (m x K) and B (n x K), original column order;
+ * dtype BF16 for an FP32 build: dC and B in BF16, FP32 dvals. */
 int swift_sddmm(const swift_matrix *A, int K, swift_dtype dtype,
                 const void *d_dC, const void *d_B, void *d_dvals,
                 swift_stream_t stream);

@@ -573,7 +573,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     if (ebase + lane < e1) {
         mc = ld_stream(a.rv_col + ebase + lane);
         if (DV && !WALK) ms = ld_stream(a.rv_src + ebase + lane);
-        mv = to_acc(ld_stream(val + ebase + lane));
+        if (DB) mv = to_acc(ld_stream(val + ebase + lane));  // dvals need no values
     }
     T acc[Q][VEC], hr[Q][VEC];
 #pragma unroll
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
                 if (ebase + lane < e1) {
                     mc = ld_stream(a.rv_col + ebase + lane);
                     if (DV && !WALK) ms = ld_stream(a.rv_src + ebase + lane);
-                    mv = to_acc(ld_stream(val + ebase + lane));
+                    if (DB) mv = to_acc(ld_stream(val + ebase + lane));
                 }
             }
             const int off = e - ebase;
@@ -908,16 +908,24 @@ static void set_g_buffer(BwArgs &a, const swift_matrix *W, int K) {
 }
 
 // dvals = SDDMM(dC, B) walking A: H = dC (per row), G = B (per entry).
+// dtype BF16 (an FP32 build): dC and B in BF16, FP32 products, sums and
+// dvals; the walk reads no values of A.
 int sddmm_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, const void *B,
                void *dvals, hipStream_t s) {
-    if (!A || K <= 0 || !dC || !B || !dvals || dtype != A->vtype) {
-        set_error("swift_sddmm: bad arguments (dtype must equal the build value type)");
+    const bool bf16 = A && dtype == SWIFT_BF16 && A->vtype == SWIFT_F32;
+    if (!A || K <= 0 || !dC || !B || !dvals || (dtype != A->vtype && !bf16)) {
+        set_error("swift_sddmm: bad arguments (dtype must equal the build value type, or be "
+                  "BF16 for an FP32 build)");
         return SWIFT_ERR_ARG;
     }
     BwArgs a = walk_args(A, K);
     a.H = dC;
     a.G = B;
     a.dvals = dvals;
+    if (bf16) {
+        set_g_buffer<__hip_bfloat16>(a, A, K);
+        return dispatch_bw<__hip_bfloat16, float, false, true, false>(a, s);
+    }
     if (dtype == SWIFT_F64) set_g_buffer<double>(a, A, K);
     else set_g_buffer<float>(a, A, K);
     return dtype == SWIFT_F64 ? dispatch_bw<double, double, false, true, false>(a, s)

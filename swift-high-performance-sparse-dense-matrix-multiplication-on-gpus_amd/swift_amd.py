@@ -423,7 +423,8 @@ class SwiftMatrix:
 
     def sddmm(self, dC, B):
         torch = _torch()
-        dvals = torch.empty(self.nnz, dtype=dC.dtype, device=dC.device)
+        odt = torch.float32 if dC.dtype == torch.bfloat16 else dC.dtype
+        dvals = torch.empty(self.nnz, dtype=odt, device=dC.device)
         check("swift_sddmm", lib().swift_sddmm(self.h, dC.shape[1], _dtype_code(dC.dtype),
                                                _ptr(dC.contiguous()), _ptr(B.contiguous()),
                                                _ptr(dvals), _stream()))

@@ -258,7 +258,8 @@ def test_backward_irregular_k(swift, oracle, cuda, name, K, dt):
                                   "banded_mixed_4000"])
 @pytest.mark.parametrize("K", [4, 32, 100, 128, 256])
 def test_backward_bf16(swift, oracle, cuda, name, K):
-    """BF16 storage of dC, B and A's values, FP32 accumulation: a product of
+    """BF16 storage of dC, B and A's values, FP32 accumulation (the fused
+    backward, and SDDMM walking A, which reads no values): a product of
     two BF16 values is exact in FP32, so against the FP64 oracle on the
     BF16-rounded operands the error is that of FP32 sums (<= 1e-5 of the
     absolute sums); against the unrounded FP32 operands it is within the
@@ -276,8 +277,10 @@ def test_backward_bf16(swift, oracle, cuda, name, K):
     B_r, dC_r, v_r = rnd(B), rnd(dC), rnd(vv)
     M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=torch.float32)
     dB, dv = M.backward_fused(dCh, Bh)
-    assert dB.dtype == torch.float32 and dv.dtype == torch.float32
+    dv_s = M.sddmm(dCh, Bh)  # SDDMM walking A itself
+    assert dB.dtype == torch.float32 and dv.dtype == torch.float32 and dv_s.dtype == torch.float32
     dB, dv = dB.cpu().numpy().astype(np.float64), dv.cpu().numpy().astype(np.float64)
+    dv_s = dv_s.cpu().numpy().astype(np.float64)
     for (vals, bb, dd), tol in (((v_r, B_r, dC_r), 1e-5),
                                 ((vv.astype(np.float64), B.astype(np.float64),
                                   dC.astype(np.float64)), 1e-2)):
@@ -287,8 +290,9 @@ def test_backward_bf16(swift, oracle, cuda, name, K):
         absv = oracle.sddmm_f64(m, n, rp, ci, np.abs(dd), np.abs(bb))
         e1 = gpu_common.rel_err(dB, dB64, absB)
         e2 = float(np.max(np.abs(dv - dv64) / (absv + 1e-30), initial=0))
-        print("tol %g: dB %.3g dvals %.3g" % (tol, e1, e2), flush=True)
-        assert e1 <= tol and e2 <= tol
+        e3 = float(np.max(np.abs(dv_s - dv64) / (absv + 1e-30), initial=0))
+        print("tol %g: dB %.3g dvals %.3g sddmm %.3g" % (tol, e1, e2, e3), flush=True)
+        assert e1 <= tol and e2 <= tol and e3 <= tol
     M.close()
 
 
