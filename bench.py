@@ -73,7 +73,7 @@ def live_traffic(storage):
             # the profiler may end with a signal after the results are written:
             # the files decide, not the exit code
             subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
-                           capture_output=True, timeout=240)
+                           capture_output=True, timeout=90)
             rows = []
             for root, _, files in os.walk(out):
                 for f in files:
@@ -391,9 +391,12 @@ def main():
         forms = [("f32", out["roofline"], reg_ms)] if not bf16 else [("bf16", out["roofline"], reg_ms)]
         if "forms" in out:
             forms.append(("bf16", out["forms"]["bf16"]["roofline"], out["forms"]["bf16"]["regular_ms"]))
+        failed = False
         for storage, roof, ms in forms:
             t0 = time.time()
-            lt = live_traffic(storage)
+            # one failed or slow profiler run ends the live measurement
+            lt = None if failed else live_traffic(storage)
+            failed = lt is None
             log("live traffic %s %.1fs: %s" % (storage, time.time() - t0, lt))
             if lt:
                 roof["traffic"] = lt.pop("traffic")
