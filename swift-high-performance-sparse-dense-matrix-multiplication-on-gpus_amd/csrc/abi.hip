@@ -373,10 +373,6 @@ extern "C" int swift_build_transpose(const swift_matrix *A, swift_matrix **AT,
 
 extern "C" int swift_sddmm(const swift_matrix *A, int K, swift_dtype dtype, const void *d_dC,
                            const void *d_B, void *d_dvals, swift_stream_t stream) {
-    if (!d_dvals || !d_B) {
-        set_error("swift_sddmm: bad arguments");
-        return SWIFT_ERR_ARG;
-    }
     SWIFT_GUARD(return sddmm_impl(A, K, dtype, d_dC, d_B, d_dvals, S(stream));)
 }
 

@@ -913,11 +913,12 @@ static void set_g_buffer(BwArgs &a, const swift_matrix *W, int K) {
 int sddmm_impl(const swift_matrix *A, int K, swift_dtype dtype, const void *dC, const void *B,
                void *dvals, hipStream_t s) {
     const bool bf16 = A && dtype == SWIFT_BF16 && A->vtype == SWIFT_F32;
-    if (!A || K <= 0 || !dC || !B || !dvals || (dtype != A->vtype && !bf16)) {
+    if (!A || K <= 0 || !dC || !B || (!dvals && A->nnz) || (dtype != A->vtype && !bf16)) {
         set_error("swift_sddmm: bad arguments (dtype must equal the build value type, or be "
                   "BF16 for an FP32 build)");
         return SWIFT_ERR_ARG;
     }
+    if (!A->nnz) return SWIFT_OK;  // no entries, no dvals
     BwArgs a = walk_args(A, K);
     a.H = dC;
     a.G = B;

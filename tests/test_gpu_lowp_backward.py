@@ -203,7 +203,7 @@ def check_backward(swift, oracle, name, K, dt, offset=0):
 # bucket of the dvals reordering (8,192 and 32,768 FP32 / 16,384 FP64 values)
 @pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "equalrows_100x70",
                                   "unsorted_rows_64x96", "narrow_50x20", "banded_3000",
-                                  "banded_mixed_4000"])
+                                  "banded_mixed_4000", "empty_10x40"])
 @pytest.mark.parametrize("K", [32, 128])
 @pytest.mark.parametrize("dt", ["f32", "f64"])
 def test_backward(swift, oracle, cuda, name, K, dt):
