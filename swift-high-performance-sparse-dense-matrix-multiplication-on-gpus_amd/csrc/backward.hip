@@ -252,8 +252,10 @@ int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t 
 //   SDDMM over A:    row i keeps H = dC[i] in registers; each entry (j, src)
 //                    gathers G = B[j]; dvals[src] = <H, G>.
 //   fused over A^T:  row j of A^T (= column j of A) keeps H = B[j]; each entry
-//                    (i, v, src) gathers G = dC[i] once and uses it twice:
-//                    dvals[src] = <H, G> and dB[j] += v * G.
+//                    (i, v) at row-view position p gathers G = dC[i] once and
+//                    uses it twice: dv[p] = <H, G> and dB[j] += v * G; dv is
+//                    then reordered into A's CSR order (k_dv_to_buckets,
+//                    k_dv_bucket_order).
 // The walked matrix's reside blocks (one column, <= 32 entries) follow with
 // the roles swapped: the column operand is loaded once, dB takes atomics.
 
@@ -273,7 +275,7 @@ struct BwArgs {
     const void *G;             // operand gathered per entry (by column)
     const void *H;             // operand per walked row
     void *dB;                  // walked-row output (fused pass) or nullptr
-    void *dvals;               // CSR-order output or nullptr
+    void *dvals;               // CSR-order output (walk order with WALK) or nullptr
     const int *split_rows;
     int nsplit;
     unsigned gbytes, gstride;  // buffer-resource path for G (0: plain loads)
