@@ -369,11 +369,21 @@ typedef struct swift_batch swift_batch; /* opaque */
  * matrices by (K, log2 mean nonzeros per row, reside share, whether the row
  * view holds the reside part), times the largest of each key over segment
  * sizes {64, 128, 256, 512} x two slot-row counts per step (the rule and 8
- * for K <= 64, 4 above) and applies the fastest to all of that key (cached
- * per process).  Operand pointers and 16-byte alignment are checked at
- * launch. */
+ * for K <= 64, 4 above), and the lane shape of half the width (K = 64..256
+ * walked as two halves of K/2 columns, each half's segments one after the
+ * other), and applies the fastest to all of that key (cached per process).
+ * Operand pointers and 16-byte alignment are checked at launch. */
 int swift_batch_create(swift_batch **out, swift_matrix *const *mats,
                        const int *Ks, int count, swift_stream_t stream);
+/* Lane widths the batch autotuner times for batches created afterwards (a
+ * process-wide setting, part of the autotuner's cache key; no counterpart
+ * in the reference).  TUNED: all of K per pass and half of it, the faster
+ * per key (the default); FULL: all of K; HALF: K >= 64 always in two
+ * halves. */
+#define SWIFT_BATCH_WIDTH_TUNED 0
+#define SWIFT_BATCH_WIDTH_FULL 1
+#define SWIFT_BATCH_WIDTH_HALF 2
+int swift_batch_set_lane_width(int mode);
 /* All C_i = A_i * B_i (row-major, original column order, F32) in one
  * persistent kernel; d_B[i], d_C[i] are device pointers. */
 int swift_batch_spmm(swift_batch *batch, const float *const *d_B,

@@ -21,6 +21,7 @@
 #include "rowview.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -111,8 +112,9 @@ __global__ __launch_bounds__(256) void k_batch_walk(const RowArgs *__restrict__ 
             const int mi = __builtin_amdgcn_readfirstlane(w.x);
             const int sid = __builtin_amdgcn_readfirstlane(w.y);
             const RowArgs &a = desc[mi];
-            // lane shape by K, slot rows by the autotuner (0: the rule)
-            switch (a.ldb | (a.slots << 12)) {
+            // lane shape by the slice width (K, or K/2 for a matrix walked in
+            // two halves), slot rows by the autotuner (0: the rule)
+            switch (a.kcount | (a.slots << 12)) {
                 case 32: batch_walk<8, BUF>(a, sid); break;
                 case 32 | (batch_alt_slots(32) << 12): batch_walk<8, BUF, batch_alt_slots(32)>(a, sid); break;
                 case 64: batch_walk<16, BUF>(a, sid); break;
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(256) void k_batch_reside(const RowArgs *__restrict_
 // (K, log2 mean nonzeros per row, reside share bucket, row view holds the
 // reside entries): matrices of one key walk the same entry sets with the
 // same lane shape.
-using TuneKey = std::tuple<int, int, int, int>;
+using TuneKey = std::tuple<int, int, int, int, int>;
 struct TuneChoice {
     int W = 128, slots = 0;
     swift_kernel kernel = SWIFT_KERNEL_DEFAULT;
@@ -155,13 +157,15 @@ struct TuneChoice {
 };
 static std::map<TuneKey, TuneChoice> g_tuned;
 static std::mutex g_tune_mu;
+// swift_batch_set_lane_width: which lane widths the batch autotuner times
+static std::atomic<int> g_batch_width{SWIFT_BATCH_WIDTH_TUNED};
 
 static TuneKey tune_key(const swift_matrix *M, int K) {
     const double mean = M->m ? (double)M->nnz / M->m : 0.0;
     const int dens = (int)std::floor(std::log2(mean + 1.0));
     const double share = M->nnz ? (double)M->reside_nnz / M->nnz : 0.0;
     const int res = share == 0.0 ? 0 : share < 0.01 ? 1 : share < 0.1 ? 2 : 3;
-    return TuneKey(K, dens, res, M->rv_all ? 1 : 0);
+    return TuneKey(K, dens, res, M->rv_all ? 1 : 0, g_batch_width);
 }
 
 // Makes c the matrix's schedule for products of K columns with elem-byte B
@@ -183,8 +187,8 @@ static int apply_choice(swift_matrix *M, int K, size_t elem, const TuneChoice &c
 // product's lane shape (slot_candidates), plus at each size the window walk
 // (LDS-staged column windows) and the lane shape of half the width
 // (width_candidates: K in two slices of narrower lane groups).  for_batch:
-// only what the batched walk runs, the rule and batch_alt_slots, with the
-// row-view walk.
+// only what the batched walk runs, the rule and batch_alt_slots, the half
+// width, with the row-view walk.
 int autotune_impl(swift_matrix *M, int K, swift_dtype dtype, swift_layout layout, const void *B,
                   void *C, hipStream_t s, bool for_batch, TuneChoice *best_out) {
     static const int sizes[] = {64, 128, 256, 512};
@@ -193,6 +197,13 @@ int autotune_impl(swift_matrix *M, int K, swift_dtype dtype, swift_layout layout
     std::vector<int> slots, widths{0};
     if (for_batch) {
         slots = {0, batch_alt_slots(K)};
+        widths = width_candidates(K, elem, true);
+        if (g_batch_width == SWIFT_BATCH_WIDTH_FULL) {
+            widths = {0};
+        } else if (g_batch_width == SWIFT_BATCH_WIDTH_HALF && widths.size() > 1) {
+            widths = {widths[1]};
+            slots.clear();  // half width only, at the rule's slot rows
+        }
     } else {
         const int vec = vec_of(elem);
         const bool aligned = K % vec == 0 && (((uintptr_t)B | (uintptr_t)C) % 16) == 0;
@@ -246,7 +257,12 @@ using namespace swift;
 struct swift_batch {
     std::vector<swift_matrix *> mats;
     std::vector<int> Ks, W, slots;  // per matrix: K, segment size, slot rows (0: rule)
+    std::vector<int> cols;          // per matrix: K/2 when walked in two halves, else 0
+    // descriptors: one per matrix (full K: split-row zeroing, reside blocks and
+    // whole-width walks), then two per matrix walked in halves (kbase 0 and
+    // K/2); half_of[j] is the matrix of descriptor count + j
     std::vector<RowArgs> desc_h;
+    std::vector<int> half_of;
     RowArgs *desc = nullptr;
     int2 *items = nullptr, *res_items = nullptr, *split = nullptr;
     int nitems = 0, nres = 0, nsplit = 0;
@@ -323,6 +339,7 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
         b->Ks.assign(Ks, Ks + count);
         b->W.assign(count, 128);
         b->slots.assign(count, 0);
+        b->cols.assign(count, 0);
         // autotune one representative per key
         std::map<TuneKey, int> rep;
         for (int i = 0; i < count; i++) {
@@ -359,6 +376,7 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
                 if (tune_key(mats[i], Ks[i]) == kv.first) {
                     b->W[i] = c.W;
                     b->slots[i] = c.slots;
+                    b->cols[i] = c.cols;
                 }
         }
         if (rc == SWIFT_OK) {
@@ -370,7 +388,8 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
             }
             for (int i = 0; i < count && rc == SWIFT_OK; i++)
                 rc = apply_choice(mats[i], Ks[i], sizeof(float),
-                                  TuneChoice{b->W[i], b->slots[i], SWIFT_KERNEL_DEFAULT, 0}, s);
+                                  TuneChoice{b->W[i], b->slots[i], SWIFT_KERNEL_DEFAULT, b->cols[i]},
+                                  s);
         }
         if (rc == SWIFT_OK) {
             // flattened work lists (host) -> device
@@ -380,10 +399,25 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
             double total = 0.0;
             for (int i = 0; i < count; i++) {
                 swift_matrix *M = mats[i];
-                for (int q = 0; q < M->nseg; q++) {
-                    items.push_back(make_int2(i, q));
-                    total += (double)b->W[i] * b->Ks[i];
-                    work.push_back(total);
+                if (b->cols[i]) {
+                    // two halves of K, each one descriptor; every segment of
+                    // the first half, then of the second, so that a stretch
+                    // of the XCD's work gathers from one half of B
+                    const int d0 = count + (int)b->half_of.size();
+                    b->half_of.push_back(i);
+                    b->half_of.push_back(i);
+                    for (int h = 0; h < 2; h++)
+                        for (int q = 0; q < M->nseg; q++) {
+                            items.push_back(make_int2(d0 + h, q));
+                            total += (double)b->W[i] * b->cols[i];
+                            work.push_back(total);
+                        }
+                } else {
+                    for (int q = 0; q < M->nseg; q++) {
+                        items.push_back(make_int2(i, q));
+                        total += (double)b->W[i] * b->Ks[i];
+                        work.push_back(total);
+                    }
                 }
                 if (!M->rv_all)  // rv_all: the walk holds the reside entries
                     for (int q = 0; q < M->numblk; q++) res.push_back(make_int2(i, q));
@@ -399,6 +433,15 @@ extern "C" int swift_batch_create(swift_batch **out, swift_matrix *const *mats, 
                 b->desc_h.push_back(row_args_of(M, b->Ks[i]));
                 b->desc_h.back().slots = b->slots[i];
                 if (!b->desc_h.back().bstride) b->buf = false;
+            }
+            for (size_t j = 0; j < b->half_of.size(); j++) {
+                const int i = b->half_of[j];
+                RowArgs h = b->desc_h[i];
+                h.kcount = b->cols[i];
+                h.kbase = (int)(j % 2) * b->cols[i];
+                h.slots = 0;
+                if (h.bstride) h.bbytes -= (unsigned)(h.kbase * sizeof(float));
+                b->desc_h.push_back(h);
             }
             auto up = [&](void **dst, const void *src, size_t bytes) -> int {
                 if (hipMalloc(dst, std::max<size_t>(bytes, 16)) != hipSuccess) {
@@ -482,6 +525,11 @@ static int batch_set_operands(swift_batch *b, const float *const *dB, float *con
             ch = true;
         }
     }
+    const size_t count = b->mats.size();
+    for (size_t j = 0; j < b->half_of.size(); j++) {
+        b->desc_h[count + j].B = b->desc_h[b->half_of[j]].B;
+        b->desc_h[count + j].C = b->desc_h[b->half_of[j]].C;
+    }
     if (ch) {
         SWIFT_HIP(hipMemcpyAsync(b->desc, b->desc_h.data(), b->desc_h.size() * sizeof(RowArgs),
                                  hipMemcpyHostToDevice, s));
@@ -550,6 +598,17 @@ extern "C" int swift_batch_graph_launch(swift_batch *b, const float *const *d_B,
         SWIFT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
     }
     SWIFT_HIP(hipGraphLaunch(b->exec, s));
+    return SWIFT_OK;
+}
+
+extern "C" int swift_batch_set_lane_width(int mode) {
+    if (mode != SWIFT_BATCH_WIDTH_TUNED && mode != SWIFT_BATCH_WIDTH_FULL &&
+        mode != SWIFT_BATCH_WIDTH_HALF) {
+        set_error("swift_batch_set_lane_width: mode must be SWIFT_BATCH_WIDTH_TUNED, _FULL or "
+                  "_HALF");
+        return SWIFT_ERR_ARG;
+    }
+    g_batch_width = mode;
     return SWIFT_OK;
 }
 

@@ -20,6 +20,7 @@ ROW_MAJOR, COL_MAJOR = 0, 1
 KERNEL_DEFAULT, KERNEL_BALANCED, KERNEL_UNBALANCED, KERNEL_KCHUNK, KERNEL_ROWSORT, KERNEL_ROWVIEW, \
     KERNEL_FLAT, KERNEL_WINDOW = range(8)
 B_SORTED, ACCUMULATE = 0x1, 0x2
+BATCH_WIDTH_TUNED, BATCH_WIDTH_FULL, BATCH_WIDTH_HALF = 0, 1, 2
 
 STATUS = {0: "OK", -1: "ERR_OPEN", -2: "ERR_BANNER", -3: "ERR_DATA", -4: "ERR_SIZE",
           -10: "ERR_ARG", -11: "ERR_HIP", -12: "ERR_ALLOC", -13: "ERR_UNSUPPORTED",
@@ -72,7 +73,7 @@ EXPORTS = (
     "swift_spmm_host", "swift_prepare_values", "swift_convert_dense", "swift_spmm_scaled",
     "swift_build_transpose", "swift_sddmm", "swift_backward_fused", "swift_update_values",
     "swift_batch_create", "swift_batch_spmm", "swift_batch_graph_launch",
-    "swift_batch_params", "swift_batch_destroy", "swift_autotune",
+    "swift_batch_params", "swift_batch_destroy", "swift_autotune", "swift_batch_set_lane_width",
 )
 
 _lib = None
@@ -481,6 +482,11 @@ def _hip():
 
 
 # -------------------------------------------------------------- batched --
+
+def batch_set_lane_width(mode):
+    """Lane widths the batch autotuner times for later batches (process-wide)."""
+    check("swift_batch_set_lane_width", lib().swift_batch_set_lane_width(mode))
+
 
 class Batch:
     """swift_batch_*: many FP32 row-major products C_i = A_i * B_i in one

@@ -215,6 +215,40 @@ def test_autotune_schedule_applied(swift, oracle, cuda, dt, K, lpe, half):
     M.close()
 
 
+@pytest.mark.parametrize("mode", ["HALF", "FULL"])
+def test_batch_lane_width(swift, oracle, cuda, mode):
+    """Every matrix of K >= 64 walked in two halves of K/2 columns (two
+    descriptors, each half's segments in turn), or none: launch and graph
+    replays against the FP64 oracle."""
+    code = getattr(swift, "BATCH_WIDTH_" + mode)
+    swift.batch_set_lane_width(code)
+    try:
+        items = make_batch(swift, 16, seed=9)
+        batch = swift.Batch([it[0] for it in items], [it[1] for it in items])
+    finally:
+        swift.batch_set_lane_width(swift.BATCH_WIDTH_TUNED)
+    for M, K, _ in items:
+        inf = M.info()
+        want = K // 2 if mode == "HALF" and K >= 64 else 0
+        assert inf.tuned_cols == want, (K, inf.tuned_cols)
+    Bs, Bn, Cs = operands(items, 6)
+    batch.spmm(Bs, Cs)
+    check_all(oracle, items, Bn, Cs)
+    for rep in range(2):
+        for c in Cs:
+            c.fill_(float("nan"))
+        batch.spmm(Bs, Cs, graph=True)
+        check_all(oracle, items, Bn, Cs)
+    batch.close()
+    for it in items:
+        it[0].close()
+
+
+def test_batch_lane_width_rejects(swift, cuda):
+    with pytest.raises(swift.SwiftError):
+        swift.batch_set_lane_width(7)
+
+
 def test_batch_schedules(swift, oracle, cuda):
     """Every batched matrix runs the schedule its key was tuned to: a segment
     size and, per K, the rule or the batched walk's other slot-row count."""
@@ -225,6 +259,7 @@ def test_batch_schedules(swift, oracle, cuda):
         assert inf.segment_items in (64, 128, 256, 512)
         assert inf.tuned_K == K and inf.tuned_kernel == swift.KERNEL_DEFAULT
         assert inf.tuned_slots in ({0, 8} if K <= 64 else {0, 4})
+        assert inf.tuned_cols in ({0, K // 2} if K >= 64 else {0})
     Bs, Bn, Cs = operands(items, 4)
     batch.spmm(Bs, Cs, graph=True)
     check_all(oracle, items, Bn, Cs)
