@@ -15,9 +15,10 @@
 //
 // Autotuner: matrices are keyed by (K, log2 mean nonzeros per row, reside
 // share bucket, whether the row view holds the reside entries); for each key
-// the largest matrix is timed with segment sizes {64, 128, 256, 512} and the
-// fastest is applied to every matrix of that key.  Results are cached per
-// process.
+// the largest matrix is timed with segment sizes {64, 128, 256, 512} x two
+// slot-row counts, and at half the lane width (K in two halves, walked as
+// two descriptors), and the fastest is applied to every matrix of that key.
+// Results are cached per process.
 #include "rowview.h"
 
 #include <algorithm>
