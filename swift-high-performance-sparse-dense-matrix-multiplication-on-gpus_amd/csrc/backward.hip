@@ -993,6 +993,11 @@ int backward_impl(const swift_matrix *AT, int K, swift_dtype dtype, const void *
             set_error("swift_backward_fused: BF16 needs swift_prepare_values(AT, BF16) first");
             return SWIFT_ERR_ARG;
         }
+        if (!AT->rv_all) {  // the BF16 walk has no reside blocks of its own
+            set_error("swift_backward_fused: BF16 needs a transposed layout "
+                      "(swift_build_transpose)");
+            return SWIFT_ERR_ARG;
+        }
         return backward_t<__hip_bfloat16, float>(AT, K, dC, B, dB, dvals, s);
     }
     return dtype == SWIFT_F64 ? backward_t<double, double>(AT, K, dC, B, dB, dvals, s)
