@@ -143,6 +143,56 @@ int csr_to_csc_with_src(int m, int n, int nnz, swift_dtype vtype, const void *d_
                         const int *d_rowptr, const int *d_colidx, void *d_cscval, int *d_cscrow,
                         int *d_cscptr, int *d_npc, int *d_cscsrc, hipStream_t s);
 
+// The plan that moves the fused backward's dvals from walk order into the
+// original matrix's CSR order (swift_matrix::dv_rank / dv_dst / dv_off).  The
+// row view of a transposed layout holds every entry once, so rv_src is a
+// permutation of the CSR positions.  Bucket order: walk positions stably
+// sorted by bucket (CSR position >> dv_log); (chunk, bucket, walk) order:
+// bucket order stably sorted by chunk (walk position >> dv_clog).
+static int build_dv_plan(swift_matrix *T, hipStream_t s) {
+    if (T->rv_nnz != T->nnz || !T->nnz) return SWIFT_OK;
+    const size_t n = (size_t)T->nnz;
+    const int lg = dv_bucket_log(T->vtype), lc = dv_chunk_log(T->vtype);
+    Buffer keys, kout, border, corder, kalt, valt, slot, tmp;
+    SWIFT_TRY(keys.alloc(4 * n));
+    SWIFT_TRY(kout.alloc(4 * n));
+    SWIFT_TRY(border.alloc(4 * n));
+    SWIFT_TRY(corder.alloc(4 * n));
+    SWIFT_TRY(kalt.alloc(4 * n));
+    SWIFT_TRY(valt.alloc(4 * n));
+    SWIFT_TRY(slot.alloc(4 * n));
+    SWIFT_TRY(tmp.alloc(radix_temp_bytes(n)));
+    hipLaunchKernelGGL(k_shift_keys<int>, dim3(g1(n)), dim3(256), 0, s, T->rv_src, n, lg,
+                       keys.as<unsigned>());
+    SWIFT_CHECK_LAUNCH();
+    SWIFT_TRY(radix_sort_pairs(keys.as<unsigned>(), nullptr, kout.as<unsigned>(),
+                               border.as<unsigned>(), kalt.as<unsigned>(), valt.as<unsigned>(),
+                               n, bits_for((unsigned)((n - 1) >> lg)), tmp.ptr, s));
+    hipLaunchKernelGGL(k_shift_keys<unsigned>, dim3(g1(n)), dim3(256), 0, s,
+                       border.as<unsigned>(), n, lc, keys.as<unsigned>());
+    SWIFT_CHECK_LAUNCH();
+    SWIFT_TRY(radix_sort_pairs(keys.as<unsigned>(), border.as<unsigned>(), kout.as<unsigned>(),
+                               corder.as<unsigned>(), kalt.as<unsigned>(), valt.as<unsigned>(),
+                               n, bits_for((unsigned)((n - 1) >> lc)), tmp.ptr, s));
+    if (hipMalloc((void **)&T->dv_rank, 2 * n) != hipSuccess ||
+        hipMalloc((void **)&T->dv_off, 2 * n) != hipSuccess ||
+        hipMalloc((void **)&T->dv_dst, 4 * n) != hipSuccess) {
+        set_error("swift_build_transpose: hipMalloc(%zu) failed", 8 * n);
+        return SWIFT_ERR_ALLOC;
+    }
+    T->dv_log = lg;
+    T->dv_clog = lc;
+    hipLaunchKernelGGL(k_invert, dim3(g1(n)), dim3(256), 0, s, border.as<unsigned>(), n,
+                       slot.as<int>());
+    hipLaunchKernelGGL(k_bucket_offsets, dim3(g1(n)), dim3(256), 0, s, border.as<unsigned>(),
+                       T->rv_src, n, lg, T->dv_off);
+    hipLaunchKernelGGL(k_chunk_plan, dim3(g1(n)), dim3(256), 0, s, corder.as<unsigned>(),
+                       slot.as<int>(), n, lc, T->dv_rank, T->dv_dst);
+    SWIFT_CHECK_LAUNCH();
+    SWIFT_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+    return SWIFT_OK;
+}
+
 int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t s) {
     const size_t vs = A->vtype == SWIFT_F64 ? 8 : 4;
     Buffer vals, cscptr, cscrow, cscval, cscsrc, npc;
@@ -181,65 +231,16 @@ int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t 
     if (T->rv_nnz)
         hipLaunchKernelGGL(k_compose, dim3(g1(T->rv_nnz)), dim3(256), 0, s, T->rv_src,
                            cscsrc.as<int>(), (size_t)T->rv_nnz);
-    // The row view holds every entry once, so rv_src is a permutation of A's
-    // CSR positions.  The fused backward writes dvals in walk order
-    // (coalesced) and moves them into CSR order in two passes: walk order ->
-    // buckets of CSR positions (a stable sort of the walk positions by
-    // bucket), then each bucket reordered in LDS.
-    if (T->rv_nnz == T->nnz && T->nnz) {
-        const size_t n = (size_t)T->nnz;
-        const int lg = dv_bucket_log(T->vtype), lc = dv_chunk_log(T->vtype);
-        Buffer keys, kout, border, corder, kalt, valt, slot, tmp;
-        SWIFT_TRY(keys.alloc(4 * n));
-        SWIFT_TRY(kout.alloc(4 * n));
-        SWIFT_TRY(border.alloc(4 * n));
-        SWIFT_TRY(corder.alloc(4 * n));
-        SWIFT_TRY(kalt.alloc(4 * n));
-        SWIFT_TRY(valt.alloc(4 * n));
-        SWIFT_TRY(slot.alloc(4 * n));
-        SWIFT_TRY(tmp.alloc(radix_temp_bytes(n)));
-        // bucket order: walk positions stably sorted by bucket
-        hipLaunchKernelGGL(k_shift_keys<int>, dim3(g1(n)), dim3(256), 0, s, T->rv_src, n, lg,
-                           keys.as<unsigned>());
-        SWIFT_CHECK_LAUNCH();
-        int rc2 = radix_sort_pairs(keys.as<unsigned>(), nullptr, kout.as<unsigned>(),
-                                   border.as<unsigned>(), kalt.as<unsigned>(),
-                                   valt.as<unsigned>(), n, bits_for((unsigned)((n - 1) >> lg)),
-                                   tmp.ptr, s);
-        // (chunk, bucket, walk) order: bucket order stably sorted by chunk
-        if (rc2 == SWIFT_OK) {
-            hipLaunchKernelGGL(k_shift_keys<unsigned>, dim3(g1(n)), dim3(256), 0, s,
-                               border.as<unsigned>(), n, lc, keys.as<unsigned>());
-            SWIFT_CHECK_LAUNCH();
-            rc2 = radix_sort_pairs(keys.as<unsigned>(), border.as<unsigned>(),
-                                   kout.as<unsigned>(), corder.as<unsigned>(),
-                                   kalt.as<unsigned>(), valt.as<unsigned>(), n,
-                                   bits_for((unsigned)((n - 1) >> lc)), tmp.ptr, s);
-        }
-        if (rc2 == SWIFT_OK &&
-            (hipMalloc((void **)&T->dv_rank, 2 * n) != hipSuccess ||
-             hipMalloc((void **)&T->dv_off, 2 * n) != hipSuccess ||
-             hipMalloc((void **)&T->dv_dst, 4 * n) != hipSuccess)) {
-            set_error("swift_build_transpose: hipMalloc(%zu) failed", 8 * n);
-            rc2 = SWIFT_ERR_ALLOC;
-        }
-        if (rc2 != SWIFT_OK) {
-            destroy_matrix(T);
-            return rc2;
-        }
-        T->dv_log = lg;
-        T->dv_clog = lc;
-        hipLaunchKernelGGL(k_invert, dim3(g1(n)), dim3(256), 0, s, border.as<unsigned>(), n,
-                           slot.as<int>());
-        hipLaunchKernelGGL(k_bucket_offsets, dim3(g1(n)), dim3(256), 0, s, border.as<unsigned>(),
-                           T->rv_src, n, lg, T->dv_off);
-        hipLaunchKernelGGL(k_chunk_plan, dim3(g1(n)), dim3(256), 0, s, corder.as<unsigned>(),
-                           slot.as<int>(), n, lc, T->dv_rank, T->dv_dst);
-        SWIFT_CHECK_LAUNCH();
-        SWIFT_HIP(hipStreamSynchronize(s));
+    rc = build_dv_plan(T, s);
+    if (rc == SWIFT_OK && hipGetLastError() != hipSuccess) {
+        set_error("swift_build_transpose: a kernel launch failed");
+        rc = SWIFT_ERR_HIP;
     }
-    SWIFT_CHECK_LAUNCH();
-    SWIFT_HIP(hipStreamSynchronize(s));
+    if (rc == SWIFT_OK && hipStreamSynchronize(s) != hipSuccess) rc = SWIFT_ERR_HIP;
+    if (rc != SWIFT_OK) {
+        destroy_matrix(T);
+        return rc;
+    }
     *out = T;
     return SWIFT_OK;
 }
