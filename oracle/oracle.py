@@ -3,8 +3,8 @@
 Loads ``oracle/liboracle.so`` (the plain-C restatement in swift_oracle.c) and,
 when present, ``oracle/_ref/libswiftref_{f64,f32}.so`` (the reference's own CPU
 headers compiled in place by oracle/Makefile).  Only tests, the smoke entry
-point and the benchmark's CPU-baseline leg import this module, always as the
-checker; the product path never does.
+point and the benchmark's CPU-baseline leg import this module, always to
+compute expected results; the product path never does.
 """
 import ctypes as C
 import os

@@ -1,8 +1,8 @@
 /*
  * swift_oracle.c -- CPU oracle for the Swift SpMM path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for tests/, the smoke entry point and
- * the CPU-baseline leg of bench.py.  The product library never links or calls
+ * TEST INFRASTRUCTURE ONLY: the expected results of tests/ and of the smoke
+ * entry point, and the CPU-baseline leg of bench.py.  The product library never links or calls
  * this file.  Every function restates a reference CPU routine and cites it.
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, so float sums are not
  * contracted into FMAs and the Swift_cpu restatement stays bitwise equal to the

@@ -3,9 +3,9 @@
  *
  * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference's
  * CPU algorithms (loader, CSR->CSC, column sort, staircase tiling, tile
- * categorisation, balance, row sort, Swift_cpu), used as the checker by the
- * tests, by the smoke entry point and by the CPU-baseline leg of the
- * benchmark. The product library never links, loads or calls it.
+ * categorisation, balance, row sort, Swift_cpu), which gives the expected
+ * results of the tests and of the smoke entry point, and the CPU-baseline leg
+ * of the benchmark. The product library never links, loads or calls it.
  *
  * Each function cites the reference file:line it restates.
  */
