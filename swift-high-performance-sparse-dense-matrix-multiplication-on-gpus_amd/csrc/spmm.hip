@@ -1,21 +1,26 @@
 // spmm.hip -- Swift SpMM kernels for gfx950 (CDNA4, 64-lane waves).
 //
-// Regular part (32-wide staircase tiles) and irregular part (reside CSC split
-// into balance blocks), for row-major and column-major dense operands.
+// SWIFT_KERNEL_DEFAULT, row-major B/C: the merge-path walk of the row view
+// (k_rowview_row over rowview.h's rowview_walk).  Every wave owns a segment
+// of W items (rows + entries), lanes run along K (LPE lanes x VEC elements
+// per entry, EPS = 64/LPE entries side by side), B rows are gathered with
+// buffer loads and each finished row is stored once; rows cut by a segment
+// boundary (and, in the fused launch, rows that reside entries add into) are
+// zeroed first and take atomics.  The reside blocks run as extra blocks of
+// the same launch when few rows receive them.  The autotuner's schedule
+// (segment size, slot rows per step, lane width, window walk) applies per K.
+// Column-major B/C: B is transposed to row-major, the same walk runs, and C
+// is transposed back.
 //
-// Row-major B/C (the fast layout).  Lanes run along K: one entry's B row is
-// LPE lanes x VEC elements (16-byte loads), so a wave holds EPS = 64/LPE
-// entries at a time.  The wave walks the entries of one work item (up to 16
-// depth tiles that share the same 32 sorted columns, so their B rows stay hot
-// in L1/L2), keeps a register accumulator per half-wave and issues the C
-// atomics only when the row changes: a format-0 tile (all 32 rows equal)
-// costs EPS row flushes instead of 32.
-//
-// Column-major B/C (the reference's Swift_GPU layout).  Lanes run along the
-// 32 entries of a tile (two tiles per wave), each k reads 32 consecutive
-// sorted columns of B (coalesced), and equal-row runs inside the tile are
-// summed with a 5-step segmented shuffle reduction before one atomic per run
-// head (format 0/2 tiles); format 1 tiles skip the reduction.
+// The reference's own orders, kept for variant parity (BALANCED, UNBALANCED,
+// KCHUNK, ROWSORT): the regular part in 32-wide staircase tiles and the
+// irregular part as reside CSC balance blocks.  Row-major: lanes along K, the
+// wave walks one work item of up to 16 depth tiles that share 32 sorted
+// columns and flushes a row's sum when the row changes.  Column-major (the
+// reference's Swift_GPU layout): lanes along the 32 entries of a tile (two
+// tiles per wave), each k reads 32 consecutive sorted columns of B, and
+// equal-row runs are summed with a 5-step segmented shuffle reduction before
+// one atomic per run head (format 0/2 tiles; format 1 tiles skip it).
 #include "rowview.h"
 
 #include <algorithm>
