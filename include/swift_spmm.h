@@ -55,8 +55,7 @@ typedef enum {
                                     through its row view (merge-path balanced), one
                                     row at a time per wave; reside part in balance
                                     blocks, or in the row view when it is at most 1%
-                                    of the entries and reaches over 1 row in 64
-                                    (swift_info.rowview_nnz = nnz)                   */
+                                    of the entries (swift_info.rowview_nnz = nnz)  */
     SWIFT_KERNEL_FLAT = 6,       /* row-major: row view walked by independent lane
                                     groups, several entries in flight whatever the
                                     row lengths; reside part as for ROWVIEW          */
@@ -264,8 +263,12 @@ int swift_permute_dense(const swift_matrix *mat, int K, swift_layout layout,
  * swift_prepare_values first and take B in that type, C in F32.
  * Column-major products with SWIFT_KERNEL_DEFAULT transpose B and C through a
  * workspace held by the handle: such calls on one handle must not overlap in
- * time (issue them on one stream, or synchronise between streams).  Row-major
- * calls share nothing and may overlap. */
+ * time (issue them on one stream, or synchronise between streams).  The
+ * row-view kernels (DEFAULT, ROWVIEW, WINDOW) keep the partial sums of rows
+ * that their segments cut in carry slots, in a workspace the handle holds per
+ * stream (allocated by the first such call on that stream, grown when a
+ * larger K needs more): calls on different streams may overlap; for HIP graph
+ * capture, make one call on the stream first or use swift_spmm_noalloc. */
 int swift_spmm(const swift_matrix *mat, int K, swift_layout layout,
                swift_dtype dtype, swift_kernel kernel, unsigned flags,
                const void *d_B, void *d_C, swift_stream_t stream);
@@ -273,13 +276,15 @@ int swift_spmm(const swift_matrix *mat, int K, swift_layout layout,
 /* swift_spmm that allocates nothing.  No counterpart in the reference: it is
  * for callers that cannot call hipMalloc (code under HIP graph capture, tight
  * serving loops, harnesses that forbid allocation).  Arguments, results and
- * errors are those of swift_spmm.  The only product that needs workspace, the
- * column-major SWIFT_KERNEL_DEFAULT (B and C transposed through it), takes it
- * from one array of SWIFT_NOALLOC_SCRATCH_BYTES per GPU that the library
- * holds from its load until the process ends; a product needing more
- * (round_up(n*K*size(B), 256) + m*K*size(C) + 256 bytes) returns
- * SWIFT_ERR_ARG.  Every call on one GPU shares that array: calls that overlap
- * in time on one GPU (other streams or threads) race. */
+ * errors are those of swift_spmm.  Workspace -- the column-major
+ * SWIFT_KERNEL_DEFAULT's transposes of B and C (round_up(n*K*size(B), 256) +
+ * m*K*size(C) + 256 bytes) and the row-view kernels' carry slots (K values of
+ * C's type per segment boundary that cuts a row, after the transposes, from a
+ * 256-byte boundary) -- comes from one array of SWIFT_NOALLOC_SCRATCH_BYTES
+ * per GPU that the library holds from its load until the process ends; a
+ * product needing more returns SWIFT_ERR_ARG.  Every call on one GPU shares
+ * that array: calls that overlap in time on one GPU (other streams or
+ * threads) race. */
 #define SWIFT_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
 int swift_spmm_noalloc(const swift_matrix *mat, int K, swift_layout layout,
                        swift_dtype dtype, swift_kernel kernel, unsigned flags,
@@ -288,12 +293,11 @@ int swift_spmm_noalloc(const swift_matrix *mat, int K, swift_layout layout,
 /* swift_spmm with HIP events recorded on `stream` between its launches, the
  * way the reference times only its kernels (src/Swift.h:298-336, where C is
  * zeroed before the timed region): kernel_ms[0] = zeroing of C, [1] = regular
- * (tile) kernels, [2] = irregular (reside) kernels.  SWIFT_KERNEL_DEFAULT
- * row-major may run the reside blocks inside the regular launch (when few rows
- * receive reside entries); [1] then includes them and [2] is only the gap
- * between two events.  SWIFT_KERNEL_ROWVIEW keeps them apart, unless the row
- * view holds the reside entries (swift_info.rowview_nnz = nnz): then every
- * row-view kernel walks them in [1].
+ * (tile) kernels, [2] = irregular (reside) kernels.  The row-view kernels
+ * zero nothing ([0] is the gap between two events); their [2] is the tail
+ * launch that adds the carry slots of cut rows onto C together with the
+ * reside blocks, or only the former when the row view holds the reside
+ * entries (swift_info.rowview_nnz = nnz) and the walk in [1] covers them.
  * Synchronises `stream`. */
 int swift_spmm_timed(const swift_matrix *mat, int K, swift_layout layout,
                      swift_dtype dtype, swift_kernel kernel, unsigned flags,
@@ -367,11 +371,12 @@ typedef struct swift_batch swift_batch; /* opaque */
  * product (test/k-32/FP32/Swift_gpu_row_major.h:264,280); here one work list
  * of (matrix, segment) items serves every matrix.  The autotuner keys
  * matrices by (K, log2 mean nonzeros per row, reside share, whether the row
- * view holds the reside part), times the largest of each key over segment
- * sizes {64, 128, 256, 512} x two slot-row counts per step (the rule and 8
- * for K <= 64, 4 above), and the lane shape of half the width (K = 64..256
- * walked as two halves of K/2 columns, each half's segments one after the
- * other), and applies the fastest to all of that key (cached per process).
+ * view holds the reside part) and times the matrices of each key as one
+ * batch over segment sizes {64, 128, 256, 512} x two slot-row counts per step
+ * (the rule and 8 for K <= 64, 4 above), and the lane shape of half the width
+ * (K = 64..256 walked as two halves of K/2 columns, each half's segments one
+ * after the other), and applies the fastest to all of that key (cached per
+ * process).  The batch holds the carry slots of every matrix's cut rows.
  * Operand pointers and 16-byte alignment are checked at launch. */
 int swift_batch_create(swift_batch **out, swift_matrix *const *mats,
                        const int *Ks, int count, swift_stream_t stream);

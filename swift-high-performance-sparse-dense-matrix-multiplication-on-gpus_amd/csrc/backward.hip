@@ -880,8 +880,8 @@ static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
 static BwArgs walk_args(const swift_matrix *W, int K) {
     BwArgs a{};
     a.m = W->m;
-    a.nseg = W->nseg;
-    a.seg = W->seg;
+    a.nseg = W->prim.nseg;
+    a.seg = W->prim.seg;
     a.rv_ptr = W->rv_ptr;
     a.rv_col = W->rv_col;
     a.rv_src = W->rv_src;
@@ -896,8 +896,8 @@ static BwArgs walk_args(const swift_matrix *W, int K) {
     a.rsrc = W->reside_src;
     a.rval = W->reside_val;
     a.perm = W->perm;
-    a.split_rows = W->split_rows;
-    a.nsplit = W->nsplit;
+    a.split_rows = W->prim.split_rows;
+    a.nsplit = W->prim.nsplit;
     a.K = K;
     return a;
 }

@@ -591,7 +591,7 @@ __global__ void k_mp_seg(int m, int R, int W, int nseg, const int *__restrict__ 
 }
 
 // A segment ending inside row r (entries of r before and after its end) makes
-// r a split row: its partial sums are added atomically onto a zeroed row.
+// r a split row, walked in pieces by several segments.
 __global__ void k_mp_split_flag(int m, int nseg, const int *__restrict__ ptr,
                                 const int2 *__restrict__ seg, int *__restrict__ flag) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -605,6 +605,38 @@ __global__ void k_mp_split_compact(int nseg, const int2 *__restrict__ seg,
                                    int *__restrict__ rows) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nseg && flag[s]) rows[idx[s]] = seg[s + 1].x;
+}
+
+// Boundary b opens a run when it cuts another row than boundary b - 1.
+__global__ void k_run_flag(int n, const int *__restrict__ rows, int *__restrict__ flag) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) flag[b] = (b == 0 || rows[b] != rows[b - 1]) ? 1 : 0;
+}
+
+// first[i] = first boundary of run i; first[nruns] = n.
+__global__ void k_run_compact(int n, const int *__restrict__ flag, const int *__restrict__ idx,
+                              int nruns, int *__restrict__ first) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n && flag[b]) first[idx[b]] = b;
+    if (b == n) first[nruns] = n;
+}
+
+// Sort key of run i: longer runs (more boundaries) first.
+__global__ void k_run_key(int nruns, int n, const int *__restrict__ first,
+                          unsigned *__restrict__ key) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nruns) key[i] = (unsigned)(n - (first[i + 1] - first[i]));
+}
+
+// fix[i] = (row, first boundary, end boundary, 0) of run order[i].
+__global__ void k_run_fix(int nruns, const unsigned *__restrict__ order,
+                          const int *__restrict__ first, const int *__restrict__ rows,
+                          int4 *__restrict__ fix) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nruns) return;
+    const int r = (int)order[i];
+    const int b0 = first[r];
+    fix[i] = make_int4(rows[b0], b0, first[r + 1], 0);
 }
 
 // ------------------------------------------------------------ row sort --
@@ -670,6 +702,12 @@ __global__ __launch_bounds__(256) void k_row_sort(int tilenum, const int *__rest
 
 // ------------------------------------------------------------ matrix build --
 
+static void free_seg_table(swift_matrix::SegTable &t) {
+    for (void *p : {(void *)t.seg, (void *)t.split_rows, (void *)t.seg_slot, (void *)t.fix})
+        if (p) (void)hipFree(p);
+    t = swift_matrix::SegTable();
+}
+
 static void free_matrix_arrays(swift_matrix *M) {
     void *ptrs[] = {M->perm, M->sortnnz, M->cptr, M->sortrowidx_tmp, M->sortval_tmp,
                     M->sortsrc_tmp, M->step_col, M->step_base, M->step_tall,
@@ -680,18 +718,16 @@ static void free_matrix_arrays(swift_matrix *M) {
                     M->blkstart, M->blkstop, M->blkptr, M->val_position, M->val_offset,
                     M->rowsorted, M->rowsort_src, M->csr_rowptr, M->csr_colidx,
                     M->sortval_lp, M->reside_val_lp, M->work, M->rv_ptr, M->rv_col,
-                    M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->seg, M->split_rows,
-                    M->rv_row, M->empty_rows, M->resrow_flag, M->reside_rows, M->dv_rank,
+                    M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->rv_row, M->empty_rows, M->resrow_flag, M->reside_rows, M->dv_rank,
                     M->dv_off, M->dv_dst};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    for (auto &t : M->cls) {
-        if (t.seg) (void)hipFree(t.seg);
-        if (t.split_rows) (void)hipFree(t.split_rows);
-        t = swift_matrix::SegTable();
-    }
+    free_seg_table(M->prim);
+    for (auto &t : M->cls) free_seg_table(t);
     M->scratch.release();
     M->dv_ws.release();
+    std::lock_guard<std::mutex> g(M->carry_mu);
+    M->carry_ws.clear();
 }
 
 template <class T>
@@ -952,42 +988,86 @@ int build_impl(swift_matrix *M, const int *d_rowptr, const int *d_colidx,
     return SWIFT_OK;
 }
 
-// Merge-path segment table of W items: (row, entry) start of every segment
-// and the rows cut by a segment boundary.
-static int build_seg_table(swift_matrix *M, int W, hipStream_t s, int2 **seg_out, int *nseg_out,
-                           int **split_out, int *nsplit_out) {
-    if (*seg_out) { (void)hipFree(*seg_out); *seg_out = nullptr; }
-    if (*split_out) { (void)hipFree(*split_out); *split_out = nullptr; }
+// Merge-path segment table of W items: (row, entry) start of every segment,
+// the rows cut by a segment boundary, the carry slot of each segment end and
+// the runs of boundaries that cut one row.
+static int build_seg_table(swift_matrix *M, int W, hipStream_t s, swift_matrix::SegTable &t) {
+    free_seg_table(t);
+    t.W = W;
     const long long items = (long long)M->m + M->rv_nnz;
     const int nseg = (int)((items + W - 1) / W);
-    *nseg_out = nseg;
-    *nsplit_out = 0;
-    SWIFT_TRY(dalloc(seg_out, (size_t)nseg + 1));
-    int2 *seg = *seg_out;
+    t.nseg = nseg;
+    SWIFT_TRY(dalloc(&t.seg, (size_t)nseg + 1));
     hipLaunchKernelGGL(k_mp_seg, dim3(grid1((size_t)nseg + 1)), dim3(256), 0, s, M->m,
-                       M->rv_nnz, W, nseg, M->rv_ptr, seg);
+                       M->rv_nnz, W, nseg, M->rv_ptr, t.seg);
     SWIFT_CHECK_LAUNCH();
-    if (nseg == 0) return dalloc(split_out, 1);
-    Buffer flag, idx, tmp;
+    SWIFT_TRY(dalloc(&t.seg_slot, (size_t)nseg + 1));
+    if (nseg == 0) {
+        SWIFT_TRY(dalloc(&t.split_rows, 1));
+        return dalloc(&t.fix, 1);
+    }
+    Buffer flag, tmp;
     SWIFT_TRY(flag.alloc(sizeof(int) * ((size_t)nseg + 1)));
-    SWIFT_TRY(idx.alloc(sizeof(int) * ((size_t)nseg + 1)));
-    SWIFT_TRY(tmp.alloc(scan_temp_bytes((size_t)nseg)));
+    SWIFT_TRY(tmp.alloc(scan_temp_bytes((size_t)nseg + 1)));
     hipLaunchKernelGGL(k_mp_split_flag, dim3(grid1(nseg)), dim3(256), 0, s, M->m, nseg, M->rv_ptr,
-                       seg, flag.as<int>());
+                       t.seg, flag.as<int>());
     SWIFT_CHECK_LAUNCH();
-    SWIFT_TRY(exclusive_scan(flag.as<int>(), idx.as<int>(), (size_t)nseg, idx.as<int>() + nseg,
+    SWIFT_TRY(exclusive_scan(flag.as<int>(), t.seg_slot, (size_t)nseg, t.seg_slot + nseg,
                              tmp.ptr, s));
     int nsplit = 0;
-    SWIFT_HIP(hipMemcpyAsync(&nsplit, idx.as<int>() + nseg, sizeof(int), hipMemcpyDeviceToHost, s));
+    SWIFT_HIP(hipMemcpyAsync(&nsplit, t.seg_slot + nseg, sizeof(int), hipMemcpyDeviceToHost, s));
     SWIFT_HIP(hipStreamSynchronize(s));
-    SWIFT_TRY(dalloc(split_out, (size_t)nsplit));
+    SWIFT_TRY(dalloc(&t.split_rows, (size_t)nsplit));
     if (nsplit) {
-        hipLaunchKernelGGL(k_mp_split_compact, dim3(grid1(nseg)), dim3(256), 0, s, nseg, seg,
-                           flag.as<int>(), idx.as<int>(), *split_out);
+        hipLaunchKernelGGL(k_mp_split_compact, dim3(grid1(nseg)), dim3(256), 0, s, nseg, t.seg,
+                           flag.as<int>(), t.seg_slot, t.split_rows);
         SWIFT_CHECK_LAUNCH();
+        // runs of equal rows in split_rows (one per cut row)
+        Buffer run, ridx;
+        SWIFT_TRY(run.alloc(sizeof(int) * ((size_t)nsplit + 1)));
+        SWIFT_TRY(ridx.alloc(sizeof(int) * ((size_t)nsplit + 1)));
+        hipLaunchKernelGGL(k_run_flag, dim3(grid1(nsplit)), dim3(256), 0, s, nsplit, t.split_rows,
+                           run.as<int>());
+        SWIFT_CHECK_LAUNCH();
+        SWIFT_TRY(exclusive_scan(run.as<int>(), ridx.as<int>(), (size_t)nsplit,
+                                 ridx.as<int>() + nsplit, tmp.ptr, s));
+        int nfix = 0;
+        SWIFT_HIP(hipMemcpyAsync(&nfix, ridx.as<int>() + nsplit, sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+        SWIFT_HIP(hipStreamSynchronize(s));
+        // run i starts at first[i]; the fix-up takes the runs longest first
+        Buffer first, key, order, alt, rtmp;
+        SWIFT_TRY(first.alloc(sizeof(int) * ((size_t)nfix + 1)));
+        hipLaunchKernelGGL(k_run_compact, dim3(grid1((size_t)nsplit + 1)), dim3(256), 0, s, nsplit,
+                           run.as<int>(), ridx.as<int>(), nfix, first.as<int>());
+        SWIFT_CHECK_LAUNCH();
+        SWIFT_TRY(key.alloc(sizeof(unsigned) * 2 * (size_t)nfix));
+        SWIFT_TRY(order.alloc(sizeof(unsigned) * (size_t)nfix));
+        SWIFT_TRY(alt.alloc(sizeof(unsigned) * 2 * (size_t)nfix));
+        SWIFT_TRY(rtmp.alloc(radix_temp_bytes((size_t)nfix)));
+        unsigned *k0 = key.as<unsigned>(), *k1 = k0 + nfix;
+        unsigned *a0 = alt.as<unsigned>(), *a1 = a0 + nfix;
+        hipLaunchKernelGGL(k_run_key, dim3(grid1(nfix)), dim3(256), 0, s, nfix, nsplit,
+                           first.as<int>(), k0);
+        SWIFT_CHECK_LAUNCH();
+        SWIFT_TRY(radix_sort_pairs(k0, nullptr, k1, order.as<unsigned>(), a0, a1, (size_t)nfix,
+                                   bits_for((unsigned)nsplit), rtmp.ptr, s));
+        SWIFT_TRY(dalloc(&t.fix, (size_t)nfix));
+        hipLaunchKernelGGL(k_run_fix, dim3(grid1(nfix)), dim3(256), 0, s, nfix,
+                           order.as<unsigned>(), first.as<int>(), t.split_rows, t.fix);
+        SWIFT_CHECK_LAUNCH();
+        // runs of more than FIX_LONG slots: sorted keys below nsplit - FIX_LONG
+        std::vector<unsigned> keys((size_t)nfix);
+        SWIFT_HIP(hipMemcpyAsync(keys.data(), k1, sizeof(unsigned) * nfix, hipMemcpyDeviceToHost, s));
+        SWIFT_HIP(hipStreamSynchronize(s));  // the temporaries are freed at the end of this block
+        t.nlong = (int)(std::lower_bound(keys.begin(), keys.end(),
+                                         (unsigned)std::max(0, nsplit - FIX_LONG)) - keys.begin());
+        t.nfix = nfix;
+    } else {
+        SWIFT_TRY(dalloc(&t.fix, 1));
     }
     SWIFT_HIP(hipStreamSynchronize(s));
-    *nsplit_out = nsplit;
+    t.nsplit = nsplit;
     return SWIFT_OK;
 }
 
@@ -995,7 +1075,7 @@ static int build_seg_table(swift_matrix *M, int W, hipStream_t s, int2 **seg_out
 // row table applies).
 int build_segments(swift_matrix *M, int W, hipStream_t s) {
     M->mp_items = W;
-    return build_seg_table(M, W, s, &M->seg, &M->nseg, &M->split_rows, &M->nsplit);
+    return build_seg_table(M, W, s, M->prim);
 }
 
 // The default tables.  A wave's segment of W items should keep enough B
@@ -1009,9 +1089,8 @@ int build_class_segments(swift_matrix *M, hipStream_t s) {
     const long long m = std::max(M->m, 1);
     const int mean = (int)((m + M->rv_nnz + m - 1) / m);
     for (int c = 0; c < SEG_CLASSES; c++) {
-        swift_matrix::SegTable &t = M->cls[c];
-        t.W = std::min(1024, std::max(by_bytes[c], 4 * mean));
-        SWIFT_TRY(build_seg_table(M, t.W, s, &t.seg, &t.nseg, &t.split_rows, &t.nsplit));
+        SWIFT_TRY(build_seg_table(M, std::min(1024, std::max(by_bytes[c], 4 * mean)), s,
+                                  M->cls[c]));
     }
     return SWIFT_OK;
 }
@@ -1021,26 +1100,12 @@ int build_class_segments(swift_matrix *M, hipStream_t s) {
 // order, followed by its reside entries.
 int build_rowview(swift_matrix *M, hipStream_t s) {
     const int m = M->m, R0 = M->regular_nnz;
-    // A small reside part (at most 1% of the entries) spread over many rows
-    // (more than 1 in 64) is walked with the regular entries: its balance
-    // blocks would each load a B row for few entries, and so many rows are
-    // too many to zero and store with atomics inside the walk's launch.
-    if (!M->rv_all && M->reside_nnz > 0 && m > 0 &&
-        (long long)M->reside_nnz * 100 <= (long long)M->nnz) {
-        Buffer fl, ix, st;
-        SWIFT_TRY(fl.alloc(sizeof(int) * ((size_t)m + 1)));
-        SWIFT_TRY(ix.alloc(sizeof(int) * ((size_t)m + 1)));
-        SWIFT_TRY(st.alloc(scan_temp_bytes((size_t)m + 1)));
-        SWIFT_TRY(fill_i32(fl.as<int>(), 0, (size_t)m + 1, s));
-        hipLaunchKernelGGL(k_mark_rows, dim3(grid1(M->reside_nnz)), dim3(256), 0, s,
-                           M->reside_cscrowidx, M->reside_nnz, fl.as<int>());
-        SWIFT_CHECK_LAUNCH();
-        SWIFT_TRY(exclusive_scan(fl.as<int>(), ix.as<int>(), (size_t)m, ix.as<int>() + m, st.ptr, s));
-        int nr = 0;
-        SWIFT_HIP(hipMemcpyAsync(&nr, ix.as<int>() + m, sizeof(int), hipMemcpyDeviceToHost, s));
-        SWIFT_HIP(hipStreamSynchronize(s));
-        M->rv_all = nr > m / 64;
-    }
+    // A small reside part (at most 1% of the entries) is walked with the
+    // regular entries: its balance blocks would each load a B row for few
+    // entries and add into finished rows with atomics, in a launch of their
+    // own after the walk.
+    if (!M->rv_all && M->reside_nnz > 0 && (long long)M->reside_nnz * 100 <= (long long)M->nnz)
+        M->rv_all = true;
     const int NRES = M->rv_all ? M->reside_nnz : 0;
     const int R = R0 + NRES;
     M->rv_nnz = R;

@@ -245,11 +245,18 @@ struct RowArgs {
     // buffer-resource path (row-view walk): bytes of B from its K-slice start,
     // and the row stride in bytes
     unsigned bbytes, bstride;
-    // row-view walk with the reside blocks in the same launch: rows flagged
-    // here are stored with atomics (nullptr: separate launches); blocks from
-    // walk_blocks on run the reside blocks
+    // row-view walk with carry slots (rowview_walk CARRY): a segment that
+    // starts inside a row stores that row's piece into carry slot
+    // seg_slot[sid - 1] (ldk values per slot, the K slice at kbase).  The
+    // fix-up (split_fixup) adds the nfix cut rows' slots onto C: fix[i] =
+    // (row, first slot, end slot), the first nlong rows (those with more
+    // than FIX_LONG slots) by a workgroup each, with atomics where rflag
+    // marks rows that reside blocks of the same launch add into.
+    void *carry;
+    const int *seg_slot;
+    const int4 *fix;
+    int nfix, nlong, ldk;
     const unsigned char *rflag;
-    int walk_blocks;
     // row-view walk: slot rows per step chosen by the autotuner (0: the rule
     // of rule_slots)
     int slots;
@@ -406,6 +413,130 @@ __device__ __forceinline__ void put_row(const RowArgs &a, int row, int sub, bool
     }
 }
 
+// Stores the piece of a cut row that a segment starts with into its carry
+// slot (scaled like put_row; no accumulation: the fix-up adds it onto C) and
+// clears the accumulators, as put_row does.
+template <class TC, int VEC, int Q, int LPE, bool MASK, bool SCALE>
+__device__ __forceinline__ void put_carry(const RowArgs &a, int slot, int sub,
+                                          TC (&acc)[Q][VEC],
+                                          const TC (*osc)[VEC] = nullptr) {
+    TC *crow = static_cast<TC *>(a.carry) + (size_t)slot * a.ldk + a.kbase;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int k0 = q * LPE * VEC + sub * VEC;
+        TC x[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; v++) {
+            if constexpr (SCALE) x[v] = acc[q][v] * (osc ? osc[q][v] : out_scale<TC>(a, k0 + v));
+            else x[v] = acc[q][v];
+        }
+        if (MASK) {
+#pragma unroll
+            for (int v = 0; v < VEC; v++)
+                if (k0 + v < a.kcount) crow[k0 + v] = x[v];
+        } else {
+            store_vec<TC, VEC>(crow + k0, x);
+        }
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[q][v] = (TC)0;
+    }
+}
+
+// Fix-up of cut row i by NT threads (one wave, or a 256-thread workgroup for
+// rows with many slots): C[row] += the row's carry slots, summed in slot
+// order, so the result does not depend on which segment finished first.
+// Threads take VW-element vectors of the row (16 bytes when K, C and the
+// slots allow, else single elements); a row of TE < NT vectors leaves P =
+// NT / TE thread groups, each summing every P-th slot, and the groups' sums
+// are added in group order (shuffles inside a wave, LDS across a
+// workgroup).  The row's values (its first piece) are read while the slots
+// load.  Rows that reside blocks of the same launch add into take atomics.
+template <class TC, int NT>
+__device__ __forceinline__ void split_fixup(const RowArgs &a, int i) {
+    static_assert(NT == 64 || NT == 256, "one wave or one 256-thread workgroup");
+    constexpr int V16 = 16 / (int)sizeof(TC);
+    constexpr int U = 8;  // slots in flight per thread
+    const int t = NT == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    const int4 f = a.fix[i];
+    const int row = f.x, b0 = f.y, b1 = f.z;
+    TC *crow = static_cast<TC *>(a.C) + (size_t)row * a.ldc;
+    const TC *carry = static_cast<const TC *>(a.carry);
+    const bool atom = a.rflag && a.rflag[row];
+    const int K = a.ldk;
+    const bool vec = (K % V16) == 0 && (a.ldc % V16) == 0 && ((uintptr_t)a.C % 16) == 0 &&
+                     ((uintptr_t)a.carry % 16) == 0;
+    const int VW = vec ? V16 : 1;
+    const int T = K / VW;                 // vectors per row
+    const int TE = T < NT ? T : NT;       // threads per group
+    const int P = NT / TE;                // groups
+    const int p = t / TE, j0 = t % TE;
+    for (int jb = 0; jb < T; jb += TE) {
+        const int j = jb + j0;
+        const bool mine = p < P && j < T;
+        TC cur[V16], acc[V16];
+#pragma unroll
+        for (int v = 0; v < V16; v++) cur[v] = acc[v] = (TC)0;
+        if (mine && p == 0 && !atom) {
+            if (vec) load_b<TC, V16, TC>(crow + (size_t)j * VW, cur);
+            else cur[0] = crow[j];
+        }
+        if (mine) {
+            int b = b0 + p;
+            if (vec) {
+                for (; b + (U - 1) * P < b1; b += U * P) {
+                    TC x[U][V16];
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        load_b<TC, V16, TC>(carry + (size_t)(b + u * P) * K + (size_t)j * VW, x[u]);
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+#pragma unroll
+                        for (int v = 0; v < V16; v++) acc[v] += x[u][v];
+                }
+            }
+            for (; b < b1; b += P) {
+                const TC *src = carry + (size_t)b * K + (size_t)j * VW;
+                if (vec) {
+                    TC x[V16];
+                    load_b<TC, V16, TC>(src, x);
+#pragma unroll
+                    for (int v = 0; v < V16; v++) acc[v] += x[v];
+                } else {
+                    acc[0] += src[0];
+                }
+            }
+        }
+        // the groups' sums in group order
+        if constexpr (NT == 64) {
+            for (int q = 1; q < P; q++)  // every lane takes part in the shuffles
+#pragma unroll
+                for (int v = 0; v < V16; v++) {
+                    const TC y = __shfl(acc[v], t + q * TE, 64);
+                    if (p == 0) acc[v] += y;
+                }
+        } else {
+            __shared__ TC part[NT * V16];
+            if (P > 1) {
+#pragma unroll
+                for (int v = 0; v < V16; v++) part[t * V16 + v] = acc[v];
+                __syncthreads();
+                if (p == 0)
+                    for (int q = 1; q < P; q++)
+#pragma unroll
+                        for (int v = 0; v < V16; v++) acc[v] += part[(q * TE + j0) * V16 + v];
+                __syncthreads();
+            }
+        }
+        if (mine && p == 0) {
+            TC *dst = crow + (size_t)j * VW;
+            for (int v = 0; v < VW; v++) {
+                if (atom) atomic_add(dst + v, acc[v]);
+                else dst[v] = cur[v] + acc[v];
+            }
+        }
+    }
+}
+
 // Block order -> logical block such that each XCD (blocks are dealt to the 8
 // XCDs round-robin) gets one contiguous range of logical blocks: neighbouring
 // segments cover neighbouring rows, whose B rows then meet in one L2.  Any
@@ -546,7 +677,7 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 // entries h, h+EPS, ... with UNROLL 16-byte loads per lane in flight; at a
 // row end the sub-groups' sums are combined with a butterfly and sub-group 0
 // stores the row.  Control flow is uniform across the wave.  Rows cut by a
-// segment boundary are split rows (zeroed beforehand) and take atomics.
+// segment boundary are split rows, stored in pieces (CARRY below).
 #ifndef SWIFT_UNROLL_NARROW
 #define SWIFT_UNROLL_NARROW 4
 #endif
@@ -568,8 +699,10 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 // whose used slots all fall inside the window reads them from LDS; any other
 // step reads B from memory as without a window.
 //
-// FLAGS: a.rflag marks rows that reside blocks of the same launch also add
-// into; the walk stores them with atomics.
+// CARRY: a segment that starts inside a row stores that row's piece into the
+// row's carry slot for the fix-up pass (split_fixup); every other row,
+// including the first piece of a row the segment end cuts, is stored into C
+// once.  Without CARRY both pieces take atomics onto a row zeroed beforehand.
 //
 // SLOTS: slot rows (B-row loads in flight per lane group) per step; 0 takes
 // rule_slots, other values come from the autotuner.
@@ -620,7 +753,7 @@ inline std::vector<int> slot_candidates(int K, size_t elem, bool aligned) {
 }
 
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF,
-          bool WIN = false, bool FLAGS = false, int SLOTS = 0>
+          bool WIN = false, bool CARRY = false, int SLOTS = 0>
 __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window win = Window{}) {
     static_assert(!WIN || (BUF && !MASK), "the window walk is a buffer-load, unmasked walk");
     constexpr int EPS = 64 / LPE;
@@ -638,16 +771,15 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
     const int r1 = en.x, e1 = en.y;
     int row = st.x, e = st.y;
     const bool carried_in = row < a.m && e > a.rv_ptr[row];
+    // carry slot of the piece this segment starts with (the boundary before it)
+    const int cslot = CARRY && carried_in ? a.seg_slot[sid - 1] : 0;
     const TV *val = static_cast<const TV *>(a.rv_val);
     const TB *B = static_cast<const TB *>(a.B);
     __amdgpu_buffer_rsrc_t rsrc;
     if constexpr (BUF) rsrc = make_rsrc(B + a.kbase, a.bbytes);
-    // staged row ends: lane l holds the end of row rbase + l (and its flag of
-    // reside entries)
+    // staged row ends: lane l holds the end of row rbase + l
     int rbase = row;
     int rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
-    int rfl_l = 0;
-    if constexpr (FLAGS) rfl_l = (rbase + lane < a.m) ? (int)a.rflag[rbase + lane] : 0;
     // staged entries: lane l holds entry ebase + l
     int ebase = e, mc = 0;
     TC mv = (TC)0;
@@ -669,7 +801,6 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
         if (row - rbase >= 64) {
             rbase = row;
             rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
-            if constexpr (FLAGS) rfl_l = (rbase + lane < a.m) ? (int)a.rflag[rbase + lane] : 0;
         }
         // wave-uniform: kept in scalar registers so the slot tests below branch
         const int rend = __builtin_amdgcn_readfirstlane(__shfl(rend_l, row - rbase, 64));
@@ -795,12 +926,21 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
             for (int q = 0; q < Q; q++)
 #pragma unroll
                 for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
-        // rows that other waves (or reside blocks) also add into take atomics
-        bool shared = last || (row == st.x && carried_in);
-        if constexpr (FLAGS) shared = shared || __builtin_amdgcn_readlane(rfl_l, row - rbase);
-        if (h == 0)
-            put_row<TC, VEC, Q, LPE, MASK, HOIST>(a, row, sub, shared, acc, HOIST ? osc : nullptr);
-        else
+        if (h == 0) {
+            if constexpr (CARRY) {
+                if (row == st.x && carried_in)
+                    put_carry<TC, VEC, Q, LPE, MASK, HOIST>(a, cslot, sub, acc,
+                                                           HOIST ? osc : nullptr);
+                else
+                    put_row<TC, VEC, Q, LPE, MASK, HOIST>(a, row, sub, false, acc,
+                                                         HOIST ? osc : nullptr);
+            } else {
+                // rows that other waves also add into take atomics
+                const bool shared = last || (row == st.x && carried_in);
+                put_row<TC, VEC, Q, LPE, MASK, HOIST>(a, row, sub, shared, acc,
+                                                     HOIST ? osc : nullptr);
+            }
+        } else
 #pragma unroll
             for (int q = 0; q < Q; q++)
 #pragma unroll
@@ -936,15 +1076,29 @@ __device__ __forceinline__ void reside_block(const RowArgs &a, int blk) {
     TC b[Q][VEC];
     load_strided<TB, TC, VEC, Q, LPE, MASK>(a, bcol, sub, b);
     const TV *rval = static_cast<const TV *>(a.rval);
-    for (int e = start + h; e < stop; e += EPS) {
-        const int row = a.rrow[e];
-        const TC val = to_acc(rval[e]);
-        TC acc[Q][VEC];
+    // the block's (row, value) pairs are loaded 64 at a time, one per lane,
+    // together with the B row, and broadcast with shuffles: the flushes that
+    // follow wait on no load
+    for (int e0 = start; e0 < stop; e0 += 64) {
+        int rl = 0;
+        TC vl = (TC)0;
+        if (e0 + lane < stop) {
+            rl = a.rrow[e0 + lane];
+            vl = to_acc(rval[e0 + lane]);
+        }
+        const int cnt = min(64, stop - e0);
+        for (int i0 = 0; i0 < cnt; i0 += EPS) {
+            const int row = __shfl(rl, i0 + h, 64);
+            const TC val = __shfl(vl, i0 + h, 64);
+            if (i0 + h < cnt) {
+                TC acc[Q][VEC];
 #pragma unroll
-        for (int q = 0; q < Q; q++)
+                for (int q = 0; q < Q; q++)
 #pragma unroll
-            for (int v = 0; v < VEC; v++) acc[q][v] = val * b[q][v];
-        flush_strided<TC, VEC, Q, LPE, MASK>(a, row, sub, acc);
+                    for (int v = 0; v < VEC; v++) acc[q][v] = val * b[q][v];
+                flush_strided<TC, VEC, Q, LPE, MASK>(a, row, sub, acc);
+            }
+        }
     }
 }
 

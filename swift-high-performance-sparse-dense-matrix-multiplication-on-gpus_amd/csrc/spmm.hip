@@ -4,10 +4,11 @@
 // (k_rowview_row over rowview.h's rowview_walk).  Every wave owns a segment
 // of W items (rows + entries), lanes run along K (LPE lanes x VEC elements
 // per entry, EPS = 64/LPE entries side by side), B rows are gathered with
-// buffer loads and each finished row is stored once; rows cut by a segment
-// boundary (and, in the fused launch, rows that reside entries add into) are
-// zeroed first and take atomics.  The reside blocks run as extra blocks of
-// the same launch when few rows receive them.  The autotuner's schedule
+// buffer loads and each finished row is stored once.  A row cut by segment
+// boundaries is stored in pieces: the piece holding its start into C, the
+// others into carry slots, which one tail launch adds onto C in boundary
+// order (k_row_tail) while its other workgroups run the reside blocks with
+// atomics onto the finished rows.  The autotuner's schedule
 // (segment size, slot rows per step, lane width, window walk) applies per K.
 // Column-major B/C: B is transposed to row-major, the same walk runs, and C
 // is transposed back.
@@ -91,39 +92,30 @@ constexpr int walk_max_waves() {
     return (sizeof(TB) == 4 && Q == 1 && LPE == 32) ? 7 : 8;
 }
 
-// Thin kernels over the device functions of rowview.h.
-// FUSE: blocks from a.walk_blocks on run the reside blocks, one wave each,
-// and the walk stores the rows they add into with atomics.
+// Thin kernels over the device functions of rowview.h.  The walk stores the
+// pieces that its segments start with into carry slots (k_row_tail adds them).
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF,
-          bool FUSE = false, int SLOTS = 0>
+          int SLOTS = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, walk_max_waves<TB, Q, LPE>())))
 void k_rowview_row(RowArgs a) {
-    if constexpr (FUSE) {
-        if (blockIdx.x >= (unsigned)a.walk_blocks) {
-            const int blk =
-                (int)(((blockIdx.x - a.walk_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
-            if (blk < a.nblk) reside_block<TV, TB, TC, VEC, Q, LPE, MASK>(a, blk);
-            return;
-        }
-    }
-    const unsigned lb = xcd_block(blockIdx.x, FUSE ? (unsigned)a.walk_blocks : gridDim.x);
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
     // wave-uniform segment id, kept in a scalar register
     const int sid = __builtin_amdgcn_readfirstlane((int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6));
     if (sid >= a.nseg) return;
-    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF, false, FUSE, SLOTS>(a, sid);
+    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF, false, true, SLOTS>(a, sid);
 }
 
 // The row-view walk with a.slots slot rows per step where the autotuner
 // chose a count other than the rule that the shape can take (buffer loads
 // only), else with the rule.
-template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF, bool FUSE>
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF>
 static void launch_view(const RowArgs &a, dim3 grid, hipStream_t s) {
     if constexpr (BUF && !MASK) {
         constexpr int RULE = rule_slots(LPE, Q, (int)sizeof(TB));
 #define SWIFT_SLOTS_CASE(N_)                                                              \
         if constexpr (slots_ok(LPE, Q, N_) && RULE != N_) {                              \
             if (a.slots == N_) {                                                         \
-                hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, BUF, FUSE, N_>), \
+                hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, BUF, N_>), \
                                    grid, dim3(256), 0, s, a);                            \
                 return;                                                                  \
             }                                                                            \
@@ -133,13 +125,9 @@ static void launch_view(const RowArgs &a, dim3 grid, hipStream_t s) {
         SWIFT_SLOTS_CASE(16)
 #undef SWIFT_SLOTS_CASE
     }
-    hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, BUF, FUSE>), grid, dim3(256),
-                       0, s, a);
+    hipLaunchKernelGGL((k_rowview_row<TV, TB, TC, VEC, Q, LPE, MASK, BUF>), grid, dim3(256), 0, s,
+                       a);
 }
-
-// Lane shapes whose walk keeps its register count with the reside blocks and
-// the row flags in the same kernel (wider shapes would lose waves per SIMD).
-constexpr bool fusable_shape(int q, int lpe, bool mask) { return !mask && q == 1 && lpe <= 32; }
 
 // Window walk.  One workgroup of WIN_WAVES waves takes WIN_WAVES consecutive
 // segments (one per wave) and first picks the B rows its entries reuse most:
@@ -275,7 +263,7 @@ __global__ __launch_bounds__(WIN_THREADS, 8) void k_rowview_win(RowArgs a) {
     }
     // wave-uniform segment id, kept in a scalar register
     const int sid = __builtin_amdgcn_readfirstlane(s0 + wave);
-    if (sid < a.nseg) rowview_walk<TV, TB, TC, VEC, Q, LPE, false, true, true>(a, sid, win);
+    if (sid < a.nseg) rowview_walk<TV, TB, TC, VEC, Q, LPE, false, true, true, true>(a, sid, win);
 }
 
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
@@ -283,6 +271,28 @@ __global__ __launch_bounds__(256) void k_reside_row(RowArgs a) {
     const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
     if (blk >= a.nblk) return;
     reside_block<TV, TB, TC, VEC, Q, LPE, MASK>(a, blk);
+}
+
+// After the row-view walk, adds the a.nfix cut rows' carry slots onto C
+// (split_fixup, all K columns): the a.nlong rows with the most slots by one
+// workgroup each, then the others by one wave each.  The workgroups after
+// them run the reside blocks of this K slice, one wave each, with atomics
+// onto the finished rows (a.nblk = 0: none).
+template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
+__global__ __launch_bounds__(256) void k_row_tail(RowArgs a) {
+    const unsigned long_blocks = (unsigned)a.nlong;
+    const unsigned fix_blocks = long_blocks + (unsigned)(a.nfix - a.nlong + 3) / 4u;
+    if (blockIdx.x < long_blocks) {
+        split_fixup<TC, 256>(a, (int)blockIdx.x);
+        return;
+    }
+    if (blockIdx.x < fix_blocks) {
+        const int w = (int)(((blockIdx.x - long_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
+        if (a.nlong + w < a.nfix) split_fixup<TC, 64>(a, a.nlong + w);
+        return;
+    }
+    const int w = (int)(((blockIdx.x - fix_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (w < a.nblk) reside_block<TV, TB, TC, VEC, Q, LPE, MASK>(a, w);
 }
 
 #ifndef SWIFT_FLAT_U
@@ -297,15 +307,13 @@ __global__ __launch_bounds__(256) void k_flat_row(RowArgs a) {
     flat_walk<TV, TB, TC, VEC, Q, LPE, MASK, U>(a, t / LPE);
 }
 
-// Zeroes C[row, 0 : K) of each listed row (rows, then rows2); one wave per row.
+// Zeroes C[row, 0 : K) of each listed row; one wave per row.
 template <class TC>
 __global__ __launch_bounds__(256) void k_zero_rows(const int *__restrict__ rows, int nrows,
-                                                   TC *__restrict__ C, int ldc, int K,
-                                                   const int *__restrict__ rows2 = nullptr,
-                                                   int nrows2 = 0) {
+                                                   TC *__restrict__ C, int ldc, int K) {
     const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (i >= nrows + nrows2) return;
-    const int r = i < nrows ? rows[i] : rows2[i - nrows];
+    if (i >= nrows) return;
+    const int r = rows[i];
     TC *c = C + (size_t)r * ldc;
     for (int k = threadIdx.x & 63; k < K; k += 64) c[k] = (TC)0;
 }
@@ -471,7 +479,40 @@ static inline void mark(int i, hipStream_t s) {
 }
 
 
-enum RowKind { ROW_TILES = 0, ROW_RESIDE = 1, ROW_VIEW = 2, ROW_FLAT = 3, ROW_WIN = 4 };
+// Caller-provided workspace of the calling thread's current product (set by
+// spmm_noalloc_impl around spmm_impl); nullptr: the matrix's own, grown on
+// demand.  g_ws_off: bytes of it already taken by an enclosing call (the
+// column-major transposes).
+struct Workspace {
+    void *ptr;
+    size_t bytes;
+};
+static thread_local const Workspace *g_ws = nullptr;
+static thread_local size_t g_ws_off = 0;
+
+// Carry slots for a row-view product: from the caller's workspace, else the
+// matrix's workspace for this stream (grown on demand).
+static int carry_workspace(const swift_matrix *M, size_t bytes, hipStream_t s, void **out) {
+    if (g_ws) {
+        const size_t off = (g_ws_off + 255) & ~(size_t)255;
+        if (off + bytes > g_ws->bytes) {
+            set_error("swift_spmm_noalloc: the product needs %zu bytes of workspace (%zu of "
+                      "them for the carry slots of its split rows), more than the %zu of the "
+                      "scratch array", off + bytes, bytes, g_ws->bytes);
+            return SWIFT_ERR_ARG;
+        }
+        *out = static_cast<char *>(g_ws->ptr) + off;
+        return SWIFT_OK;
+    }
+    std::lock_guard<std::mutex> g(M->carry_mu);
+    auto &slot = M->carry_ws[s];
+    if (!slot) slot.reset(new Buffer());
+    SWIFT_TRY(slot->alloc(bytes));
+    *out = slot->ptr;
+    return SWIFT_OK;
+}
+
+enum RowKind { ROW_TILES = 0, ROW_RESIDE = 1, ROW_VIEW = 2, ROW_FLAT = 3, ROW_WIN = 4, ROW_TAIL = 5 };
 
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
 static void launch_row_shape(const RowArgs &a, RowKind kind, hipStream_t s) {
@@ -499,25 +540,20 @@ static void launch_row_shape(const RowArgs &a, RowKind kind, hipStream_t s) {
         return;
     }
     if (kind == ROW_VIEW) {
-        RowArgs w = a;
-        w.walk_blocks = (int)ceil_div((size_t)a.nseg * 64, 256);
-        if constexpr (fusable_shape(Q, LPE, MASK)) {
-            if (a.rflag) {
-                // the reside blocks run as extra blocks of this launch
-                const dim3 grid((unsigned)(w.walk_blocks + ceil_div((size_t)a.nblk * 64, 256)));
-                if (a.bstride)
-                    launch_view<TV, TB, TC, VEC, Q, LPE, MASK, true, true>(w, grid, s);
-                else
-                    launch_view<TV, TB, TC, VEC, Q, LPE, MASK, false, true>(w, grid, s);
-                return;
-            }
-        }
         if (!a.nseg) return;
-        const dim3 grid((unsigned)w.walk_blocks);
+        const dim3 grid(ceil_div((size_t)a.nseg * 64, 256));
         if (a.bstride)
-            launch_view<TV, TB, TC, VEC, Q, LPE, MASK, true, false>(w, grid, s);
+            launch_view<TV, TB, TC, VEC, Q, LPE, MASK, true>(a, grid, s);
         else
-            launch_view<TV, TB, TC, VEC, Q, LPE, MASK, false, false>(w, grid, s);
+            launch_view<TV, TB, TC, VEC, Q, LPE, MASK, false>(a, grid, s);
+        return;
+    }
+    if (kind == ROW_TAIL) {
+        const size_t blocks = (size_t)a.nlong + ceil_div((size_t)(a.nfix - a.nlong) * 64, 256) +
+                              ceil_div((size_t)a.nblk * 64, 256);
+        if (blocks)
+            hipLaunchKernelGGL((k_row_tail<TV, TB, TC, VEC, Q, LPE, MASK>), dim3((unsigned)blocks),
+                               dim3(256), 0, s, a);
         return;
     }
     const unsigned items = kind == ROW_TILES ? (unsigned)a.nwork : (unsigned)a.nblk;
@@ -579,11 +615,9 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.m = M->m;
     // the default table of the B row size, unless the size was set
     const swift_matrix::SegTable &tb = M->cls[seg_class((size_t)K * sizeof(TB), sizeof(TB))];
-    const bool dflt = !M->user_items && tb.seg;
-    a.nseg = dflt ? tb.nseg : M->nseg;
-    a.seg = dflt ? tb.seg : M->seg;
-    const int nsplit = dflt ? tb.nsplit : M->nsplit;
-    const int *split_rows = dflt ? tb.split_rows : M->split_rows;
+    const swift_matrix::SegTable &st = !M->user_items && tb.seg ? tb : M->prim;
+    a.nseg = st.nseg;
+    a.seg = st.seg;
     a.rv_ptr = M->rv_ptr;
     a.rv_col = M->rv_col;
     a.rv_val = rv_val;
@@ -611,32 +645,33 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     const bool winw = kernel == SWIFT_KERNEL_WINDOW;
     const bool view = kernel == SWIFT_KERNEL_DEFAULT || kernel == SWIFT_KERNEL_ROWVIEW || flat ||
                       winw;
-    // DEFAULT: the reside blocks join the walk's launch when few rows (<= 1 in
-    // 64) receive reside entries and every K slice has a lane shape that keeps
-    // its register count (fusable_shape); those rows are zeroed with the split
-    // rows and stored with atomics by the walk.  With many such rows the
-    // atomics would cost the walk more than the launch saves.  ROWVIEW keeps
-    // the reside blocks in a launch of their own (the reference's phases).
-    bool fusable = true;
-    for (auto &sl : slices) fusable = fusable && fusable_shape(sl.second.q, sl.second.lpe, sl.second.mask);
-    const bool fused = kernel == SWIFT_KERNEL_DEFAULT && fusable && a.nblk > 0 &&
-                       !M->rv_all && M->resrow_flag && M->nreside_rows <= M->m / 64;
-    const int nres_rows = fused ? M->nreside_rows : 0;
-    a.rflag = fused ? M->resrow_flag : nullptr;
-    if (view) {
-        if (!a.accumulate && nsplit + nres_rows) {
-            hipLaunchKernelGGL(k_zero_rows<TC>,
-                               dim3(ceil_div((size_t)(nsplit + nres_rows) * 64, 256)), dim3(256),
-                               0, s, split_rows, nsplit, (TC *)C, K, K,
-                               (const int *)M->reside_rows, nres_rows);
+    // The row-view walks (not FLAT) store the pieces of rows cut by a segment
+    // boundary into carry slots; after the walk one tail launch adds them
+    // onto C (k_row_tail) and runs the reside blocks, which add onto finished
+    // rows with atomics.  The flat walk zeroes its split and empty rows first
+    // and stores split rows with atomics.
+    const bool carry = view && !flat;
+    a.fix = st.fix;
+    a.seg_slot = st.seg_slot;
+    a.nfix = carry ? st.nfix : 0;
+    a.nlong = carry ? st.nlong : 0;
+    a.ldk = K;
+    if (carry && st.nsplit)
+        SWIFT_TRY(carry_workspace(M, (size_t)st.nsplit * K * sizeof(TC), s, &a.carry));
+    const bool reside = a.nblk > 0 && !(view && M->rv_all);  // rv_all: the walk holds them
+    a.rflag = reside ? M->resrow_flag : nullptr;
+    if (flat) {
+        if (!a.accumulate && st.nsplit) {
+            hipLaunchKernelGGL(k_zero_rows<TC>, dim3(ceil_div((size_t)st.nsplit * 64, 256)),
+                               dim3(256), 0, s, st.split_rows, st.nsplit, (TC *)C, K, K);
             SWIFT_CHECK_LAUNCH();
         }
-        if (flat && !a.accumulate && M->nempty) {
+        if (!a.accumulate && M->nempty) {
             hipLaunchKernelGGL(k_zero_rows<TC>, dim3(ceil_div((size_t)M->nempty * 64, 256)),
                                dim3(256), 0, s, M->empty_rows, M->nempty, (TC *)C, K, K);
             SWIFT_CHECK_LAUNCH();
         }
-    } else if (!a.accumulate) {
+    } else if (!view && !a.accumulate) {
         SWIFT_TRY(fill_bytes_zero(C, (size_t)M->m * K * sizeof(TC), s));
     }
     mark(1, s);
@@ -651,12 +686,20 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
         SWIFT_TRY((launch_row<TV, TB, TC>(a, t, kind, s)));
     }
     mark(2, s);
+    const int nblk = a.nblk;
+    bool first = true;
     for (auto &sl : slices) {
-        if (fused || (view && M->rv_all)) break;  // rv_all: the walk holds the reside entries
         a.kbase = sl.first;
         const RowTarget &t = sl.second;
         a.kcount = t.mask ? std::min(64, K - sl.first) : t.vec * t.lpe * t.q;
-        SWIFT_TRY((launch_row<TV, TB, TC>(a, t, ROW_RESIDE, s)));
+        a.nblk = reside ? nblk : 0;
+        if (carry && first) {
+            // fix-up of the cut rows (all of K) with this slice's reside blocks
+            SWIFT_TRY((launch_row<TV, TB, TC>(a, t, ROW_TAIL, s)));
+        } else if (a.nblk) {
+            SWIFT_TRY((launch_row<TV, TB, TC>(a, t, ROW_RESIDE, s)));
+        }
+        first = false;
     }
     mark(3, s);
     return SWIFT_OK;
@@ -801,14 +844,6 @@ static int scratch_address(void **out) {
     return SWIFT_OK;
 }
 
-// Caller-provided workspace of the calling thread's current product (set by
-// spmm_noalloc_impl around spmm_impl); nullptr: the matrix's own, grown on
-// demand.
-struct Workspace {
-    void *ptr;
-    size_t bytes;
-};
-static thread_local const Workspace *g_ws = nullptr;
 
 // Column-major C = A*B.  DEFAULT: B is transposed into the matrix's
 // workspace, the row-view product runs row-major, and C is transposed back
@@ -838,8 +873,12 @@ static int spmm_col(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
         TB *Bt = static_cast<TB *>(ws);
         TC *Ct = reinterpret_cast<TC *>(static_cast<char *>(ws) + bB);
         SWIFT_TRY(transpose<TB>(static_cast<const TB *>(B), Bt, K, M->n, false, s));
-        SWIFT_TRY((spmm_row<TV, TB, TC>(M, K, SWIFT_KERNEL_DEFAULT, flags & ~SWIFT_ACCUMULATE, Bt,
-                                        Ct, sval, rval, rv_val, kscale, gscale, s)));
+        // the row-major product's carry slots follow the transposes' workspace
+        g_ws_off = bB + bC + 256;
+        const int rc = spmm_row<TV, TB, TC>(M, K, SWIFT_KERNEL_DEFAULT, flags & ~SWIFT_ACCUMULATE,
+                                            Bt, Ct, sval, rval, rv_val, kscale, gscale, s);
+        g_ws_off = 0;
+        SWIFT_TRY(rc);
         return transpose<TC>(Ct, static_cast<TC *>(C), M->m, K, accumulate, s);
     }
     if (!accumulate) SWIFT_TRY(fill_bytes_zero(C, (size_t)M->m * K * sizeof(TC), s));

@@ -8,6 +8,9 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 
 #include "swift_spmm.h"
@@ -49,6 +52,10 @@ inline size_t value_size(swift_dtype t) {
 }
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Cut rows with more carry slots than this are fixed up by a 256-thread
+// workgroup (8 groups of 32 threads at K = 128 FP32), the others by one wave.
+constexpr int FIX_LONG = 32;
 
 // Segment-table class of a product whose B rows are row_bytes long, of
 // elements elem bytes wide.
@@ -202,20 +209,38 @@ struct swift_matrix {
     unsigned char *resrow_flag = nullptr;  // m bytes, 1 = row has reside entries
     int *reside_rows = nullptr;            // those rows, ascending
     int nreside_rows = 0;
-    // merge-path split of (rows + regular entries) into segments of mp_items
-    int mp_items = 0, nseg = 0, nsplit = 0;
-    int2 *seg = nullptr;         // nseg+1 (row, entry) segment starts
-    int *split_rows = nullptr;   // rows whose entries span a segment boundary
+    // Merge-path split of (rows + regular entries) into segments of W items.
+    // A boundary that falls inside a row cuts it into pieces: the piece
+    // holding the row's start is stored into C like any row, every later
+    // piece (it starts its segment) into the carry slot of the boundary it
+    // starts at, and a fix-up pass adds a row's carry slots onto it in order.
+    //   split_rows[b]: row cut by boundary b (boundaries in segment order, so
+    //                  one row's boundaries are consecutive)
+    //   seg_slot[s]:   boundary index of the end of segment s, where it cuts
+    //                  a row (exclusive scan of the cut flags)
+    //   fix[i]:        (row, first boundary, end boundary, 0) of the i-th cut
+    //                  row, rows with the most boundaries first
+    struct SegTable {
+        int W = 0, nseg = 0, nsplit = 0, nfix = 0;
+        int nlong = 0;               // cut rows of more than FIX_LONG slots (first in fix)
+        int2 *seg = nullptr;         // nseg+1 (row, entry) segment starts
+        int *split_rows = nullptr;
+        int *seg_slot = nullptr;
+        int4 *fix = nullptr;
+    };
+    // the table of mp_items items (swift_set_segment_items, the autotuner)
+    int mp_items = 0;
+    SegTable prim;
     // Default tables by the bytes of one B row (K * element size) gathered
     // per entry (swift::seg_class) -- used unless the segment size was set
     // explicitly (user_items != 0).
-    struct SegTable {
-        int W = 0, nseg = 0, nsplit = 0;
-        int2 *seg = nullptr;
-        int *split_rows = nullptr;
-    };
     SegTable cls[swift::SEG_CLASSES];
     int user_items = 0;
+    // Carry slots of the row-view walks (nsplit rows of K values), one
+    // workspace per stream so that products on different streams do not
+    // share one; grown on demand.
+    mutable std::mutex carry_mu;
+    mutable std::map<hipStream_t, std::unique_ptr<swift::Buffer>> carry_ws;
     // autotuner choice (swift_autotune, swift_batch_create) for products of
     // tuned_K columns with tuned_elem-byte B elements: the walk that
     // SWIFT_KERNEL_DEFAULT runs (the row-view walk or the window walk), the

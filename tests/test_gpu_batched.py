@@ -61,8 +61,14 @@ def test_batch_matches_oracle_and_separate(swift, oracle, cuda):
     print("batch of", len(items), flush=True)
     batch = swift.Batch([it[0] for it in items], [it[1] for it in items])
     p = batch.params()
-    lo = sum(-(-(it[0].info().m + it[0].info().rowview_nnz) // 512) for it in items)
-    hi = sum(-(-(it[0].info().m + it[0].info().rowview_nnz) // 64) for it in items)
+    # segments of 64..512 items; a matrix walked in two halves of K has each
+    # segment twice
+    def units(it, W):
+        inf = it[0].info()
+        return -(-(inf.m + inf.rowview_nnz) // W) * (2 if inf.tuned_cols else 1)
+
+    lo = sum(units(it, 512) for it in items)
+    hi = sum(units(it, 64) for it in items)
     assert lo <= p["work_units"] <= hi
     assert p["depth"] in (64, 128, 256, 512) and p["block_threads"] == 256
     Bs, Bn, Cs = operands(items, 1)

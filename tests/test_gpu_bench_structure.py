@@ -1,15 +1,11 @@
-"""The default row-major launch with the reside blocks inside the walk's launch.
+"""Every product form on the benchmark matrix's structure at 1/20 scale.
 
-The default kernel runs the reside (irregular) blocks as extra blocks of the
-row-view walk when few rows (at most 1 in 64) receive reside entries and the
-lane shape allows it; those rows are zeroed first and stored with atomics.
-The small parity matrices all have more reside rows than that, so this file
-uses the benchmark matrix's structure at 1/20 scale (50,000 rows, 804,644
-nonzeros, 363 rows with reside entries), checks that it qualifies, and runs
-every product form through it against the FP64 oracle: FP32 at K=32/128
-(plain and accumulating into NaN-free and NaN-prefilled outputs), BF16, F16
-with column scales, column-major B, and SDDMM (whose walk also runs its
-reside blocks in one launch)."""
+50,000 rows, 804,644 nonzeros: even rows banded, odd rows with Zipf lengths
+(153 rows of more than 256 entries, the longest 13,831), so the default
+segment tables cut many rows (their pieces go through the carry slots and
+the fix-up), and 420 reside entries (under 1%), which the row view holds.  Checked against the FP64 oracle: FP32
+at K=32/128 (plain and accumulating into NaN-free and NaN-prefilled outputs),
+BF16, F16 with column scales, column-major B, and SDDMM."""
 import numpy as np
 import pytest
 
@@ -27,9 +23,19 @@ def matrix(swift, oracle):
     return _MAT["d"], _MAT["reside_rows"]
 
 
-def test_matrix_takes_fused_path(swift, oracle, cuda):
+def test_matrix_structure(swift, oracle, cuda):
+    """Rows longer than the default segments (the row view holds 17 items per
+    row on average, so segments of 96 or more) and a reside part of under 1%
+    of the entries."""
+    import torch
     d, rows = matrix(swift, oracle)
-    assert 0 < len(rows) <= d["m"] // 64, (len(rows), d["m"])
+    lens = np.diff(d["rowptr"])
+    assert lens.max() > 256 and (lens > 256).sum() > 100, (lens.max(), (lens > 256).sum())
+    M = swift.SwiftMatrix(d["m"], d["n"], d["rowptr"], d["colidx"], d["val"].astype(np.float32),
+                          dtype=torch.float32)
+    inf = M.info()
+    assert 0 < inf.reside_nnz * 100 <= d["nnz"] and inf.rowview_nnz == d["nnz"]
+    M.close()
 
 
 def _setup(swift, oracle, K, seed=0):
@@ -48,21 +54,21 @@ def _err(C, C64, absC):
 
 
 @pytest.mark.parametrize("K", [32, 128])
-def test_fused_f32(swift, oracle, cuda, K):
+def test_structure_f32(swift, oracle, cuda, K):
     import torch
     d, M, B, C64, absC = _setup(swift, oracle, K)
     Bt = torch.tensor(B, device="cuda")
     C = torch.full((d["m"], K), float("nan"), device="cuda")
     M.spmm(Bt, out=C)
     err = _err(C.cpu().numpy(), C64, absC)
-    print("fused f32 K", K, "err", err, flush=True)
+    print("bench structure f32 K", K, "err", err, flush=True)
     assert err <= 1e-5
     # C += A*B: split and reside rows take atomics onto the existing values
     C0 = np.random.default_rng(1).uniform(-1, 1, (d["m"], K)).astype(np.float32)
     C = torch.tensor(C0, device="cuda")
     M.spmm(Bt, out=C, flags=swift.ACCUMULATE)
     err = _err(C.cpu().numpy(), C64 + C0, absC + np.abs(C0))
-    print("fused f32 accumulate K", K, "err", err, flush=True)
+    print("bench structure f32 accumulate K", K, "err", err, flush=True)
     assert err <= 1e-5
     # the separate-launch kernel gives the same product
     Cr = M.spmm(Bt, kernel=swift.KERNEL_ROWVIEW)
@@ -70,7 +76,7 @@ def test_fused_f32(swift, oracle, cuda, K):
     M.close()
 
 
-def test_fused_lowp(swift, oracle, cuda):
+def test_structure_lowp(swift, oracle, cuda):
     import torch
     K = 128
     d, M, B, C64, absC = _setup(swift, oracle, K)
@@ -79,31 +85,31 @@ def test_fused_lowp(swift, oracle, cuda):
     C = torch.full((d["m"], K), float("nan"), device="cuda")
     M.spmm(swift.convert_dense(Bt, swift.BF16), out=C, dtype=swift.BF16)
     err = _err(C.cpu().numpy(), C64, absC)
-    print("fused bf16 err", err, flush=True)
+    print("bench structure bf16 err", err, flush=True)
     assert err <= 1e-2
     M.prepare_values(swift.F16)
     Bh, sc = swift.convert_dense(Bt, swift.F16, col_scale=True)
     C = torch.full((d["m"], K), float("nan"), device="cuda")
     M.spmm_scaled(Bh, sc, out=C)
     err = _err(C.cpu().numpy(), C64, absC)
-    print("fused f16 err", err, flush=True)
+    print("bench structure f16 err", err, flush=True)
     assert err <= 1e-2
     M.close()
 
 
-def test_fused_col_major(swift, oracle, cuda):
+def test_structure_col_major(swift, oracle, cuda):
     import torch
     K = 128
     d, M, B, C64, absC = _setup(swift, oracle, K)
     Bt = torch.tensor(B.T.copy(), device="cuda")
     C = M.spmm(Bt, layout=swift.COL_MAJOR).cpu().numpy().T
     err = _err(C, C64, absC)
-    print("fused col-major err", err, flush=True)
+    print("bench structure col-major err", err, flush=True)
     assert err <= 1e-5
     M.close()
 
 
-def test_fused_sddmm(swift, oracle, cuda):
+def test_structure_sddmm(swift, oracle, cuda):
     import torch
     K = 128
     d, M, B, _, _ = _setup(swift, oracle, K)
@@ -114,6 +120,6 @@ def test_fused_sddmm(swift, oracle, cuda):
                            np.abs(dC).astype(np.float64), np.abs(B).astype(np.float64))
     dv = M.sddmm(torch.tensor(dC, device="cuda"), torch.tensor(B, device="cuda")).cpu().numpy()
     err = float(np.max(np.abs(dv.astype(np.float64) - ref) / (den + 1e-30)))
-    print("fused sddmm err", err, flush=True)
+    print("bench structure sddmm err", err, flush=True)
     assert err <= 1e-5
     M.close()

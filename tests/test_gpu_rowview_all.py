@@ -1,14 +1,14 @@
 """Row views that hold the reside entries.
 
-A matrix whose reside part is at most 1% of its entries and reaches more than
-1 row in 64 has its reside entries walked with the regular ones (regular, then
-reside, within each row), with no reside launch.  The Chung-Lu graph at 1/4
-scale (50,000 nodes, 2,474,324 nonzeros, 22,484 reside entries over 12,808
-rows) qualifies; the benchmark matrix at 1/20 scale (420 reside entries over
-363 of 50,000 rows) does not.  The test checks the choice from the oracle's
-format, then runs every row-view product form through the graph against the
-FP64 oracle: FP32 under each row-view kernel (plain and accumulating), BF16,
-column-major B, SDDMM and the fused backward."""
+A matrix whose reside part is at most 1% of its entries has its reside
+entries walked with the regular ones (regular, then reside, within each row),
+with no reside blocks.  The Chung-Lu graph at 1/4 scale (50,000 nodes,
+2,474,324 nonzeros, 22,484 reside entries) and the benchmark matrix at 1/20
+scale (420 reside entries) qualify; tall_2000x64 (12% of its entries reside)
+does not.  The test checks the choice from the oracle's format, then runs
+every row-view product form through the graph against the FP64 oracle: FP32
+under each row-view kernel (plain and accumulating), BF16, column-major B,
+SDDMM and the fused backward."""
 import numpy as np
 import pytest
 
@@ -34,17 +34,25 @@ def _matrix(swift, d):
 
 
 def test_choice_of_row_view(swift, oracle, cuda):
+    import torch
     d, reside = graph(swift, oracle)
-    rows = len(np.unique(reside))
-    assert 0 < len(reside) * 100 <= d["nnz"] and rows > d["m"] // 64, (len(reside), rows)
+    assert 0 < len(reside) * 100 <= d["nnz"], len(reside)
     M = _matrix(swift, d)
     inf = M.info()
     assert inf.reside_nnz == len(reside)
     assert inf.rowview_nnz == d["nnz"] == inf.regular_nnz + inf.reside_nnz
     M.close()
-    # few reside rows: the row view keeps the regular entries only
     c = swift.generate(2, scale=0.05, seed=0)
     M = _matrix(swift, c)
+    inf = M.info()
+    assert 0 < inf.reside_nnz * 100 <= c["nnz"] and inf.rowview_nnz == c["nnz"]
+    M.close()
+    # a reside part over 1% keeps its balance blocks: the row view holds the
+    # regular entries only
+    m, n, rp, ci, v = gpu_common.cases()["tall_2000x64"]
+    f = oracle.build(m, n, rp, ci, np.asarray(v, np.float64))
+    assert len(f["reside_cscrowidx"]) * 100 > len(ci)
+    M = swift.SwiftMatrix(m, n, rp, ci, np.asarray(v, np.float32), dtype=torch.float32)
     inf = M.info()
     assert inf.reside_nnz > 0 and inf.rowview_nnz == inf.regular_nnz
     M.close()
