@@ -209,9 +209,9 @@ int swift_row_idx_sort(swift_matrix *mat, swift_stream_t stream);
 
 /* Schedule knob of the row-view kernels: items (rows + entries) per
  * merge-path segment, i.e. per wave.  By default the size follows the bytes
- * of one B row (64 items for rows of 1 KiB or more, 96 from 512 B, 192 / 128
- * from 256 B for 16-bit / wider elements, 256 below), and is at least four
- * mean rows of the matrix; a call here pins one size for every product of
+ * of one B row (48 items for rows of 1 KiB or more, 64 from 512 B, 128 from
+ * 256 B, 256 below), and is at least three mean rows of the matrix; a call
+ * here pins one size for every product of
  * the matrix.  Rebuilds the segment table on the GPU; the format
  * itself is unchanged.  The batch autotuner (swift_batch_create) and
  * swift_autotune set it per matrix.  No counterpart in the reference. */

@@ -1079,17 +1079,19 @@ int build_segments(swift_matrix *M, int W, hipStream_t s) {
 }
 
 // The default tables.  A wave's segment of W items should keep enough B
-// bytes in flight without cutting many rows: W is the size measured best on
-// the 1M-row benchmark matrix for the class's B rows (64 items for >= 1 KiB
-// rows, 96 for >= 512 B, 192 / 128 for 256..511 B of 16-bit / wider
-// elements, 256 for narrower rows), raised to 4 mean rows (items per row) so
-// that long rows are rarely split across segments.
+// bytes in flight and leave enough segments to fill the device: W is the
+// size measured best on the 1M-row benchmark matrix for the class's B rows
+// (48 items for >= 1 KiB rows, 64 for >= 512 B, 128 for 256..511 B, 256 for
+// narrower rows; FP32 K=128 64 items 0.719 ms against 96 items 0.744 ms,
+// BF16 K=128 128 items 0.416 ms against 192 items 0.447 ms), raised to 3
+// mean rows (items per row), which a matrix of long rows (the 200k-node
+// graph: 51 items per row) prefers.  A cut row costs one carry slot.
 int build_class_segments(swift_matrix *M, hipStream_t s) {
-    static const int by_bytes[SEG_CLASSES] = {64, 96, 192, 128, 256};
+    static const int by_bytes[SEG_CLASSES] = {48, 64, 128, 128, 256};
     const long long m = std::max(M->m, 1);
     const int mean = (int)((m + M->rv_nnz + m - 1) / m);
     for (int c = 0; c < SEG_CLASSES; c++) {
-        SWIFT_TRY(build_seg_table(M, std::min(1024, std::max(by_bytes[c], 4 * mean)), s,
+        SWIFT_TRY(build_seg_table(M, std::min(1024, std::max(by_bytes[c], 3 * mean)), s,
                                   M->cls[c]));
     }
     return SWIFT_OK;
