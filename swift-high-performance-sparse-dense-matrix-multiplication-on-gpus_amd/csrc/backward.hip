@@ -277,8 +277,13 @@ struct BwArgs {
     const void *H;             // operand per walked row
     void *dB;                  // walked-row output (fused pass) or nullptr
     void *dvals;               // CSR-order output (walk order with WALK) or nullptr
-    const int *split_rows;
-    int nsplit;
+    // dB rows cut by segment boundaries: a segment that starts inside a row
+    // stores that row's piece into carry slot seg_slot[sid - 1] (K values per
+    // slot); k_fix_rows adds a cut row's slots onto it after the walk
+    void *carry;
+    const int *seg_slot;
+    const int4 *fix;
+    int nfix, nlong;
     unsigned gbytes, gstride;  // buffer-resource path for G (0: plain loads)
     // dvals alone: the reside blocks run as blocks walk_blocks.. of the
     // walk's launch
@@ -563,6 +568,8 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     const int r1 = en.x, e1 = en.y;
     int row = st.x, e = st.y;
     const bool carried_in = row < a.m && e > a.rv_ptr[row];
+    // carry slot of the dB piece this segment starts with
+    const int cslot = DB && carried_in ? a.seg_slot[sid - 1] : 0;
     const TS *val = static_cast<const TS *>(a.rv_val);
     const TS *G = static_cast<const TS *>(a.G) + a.kbase;
     const TS *H = static_cast<const TS *>(a.H) + a.kbase;
@@ -687,21 +694,18 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
 #pragma unroll
                 for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
         if (h == 0) {
-            T *drow = static_cast<T *>(a.dB) + (size_t)row * a.K + a.kbase;
-            const bool split = last || (row == st.x && carried_in);
+            // the piece of a row this segment starts inside goes to its carry
+            // slot; every other row (and the first piece of a cut row) to dB
+            T *drow = (row == st.x && carried_in)
+                          ? static_cast<T *>(a.carry) + (size_t)cslot * a.K + a.kbase
+                          : static_cast<T *>(a.dB) + (size_t)row * a.K + a.kbase;
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const int k0 = q * LPE * VEC + sub * VEC;
                 if (MASK) {
 #pragma unroll
                     for (int v = 0; v < VEC; v++)
-                        if (k0 + v < a.kcount) {
-                            if (split) atomic_add_bw(drow + k0 + v, acc[q][v]);
-                            else drow[k0 + v] = acc[q][v];
-                        }
-                } else if (split) {
-#pragma unroll
-                    for (int v = 0; v < VEC; v++) atomic_add_bw(drow + k0 + v, acc[q][v]);
+                        if (k0 + v < a.kcount) drow[k0 + v] = acc[q][v];
                 } else {
                     stv<T, VEC>(drow + k0, acc[q]);
                 }
@@ -789,14 +793,6 @@ __global__ __launch_bounds__(1024) void k_dv_bucket_order(T *__restrict__ d,
     }
 }
 
-template <class T>
-__global__ void k_zero_rows_bw(const int *__restrict__ rows, int nrows, T *__restrict__ D, int K) {
-    const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (i >= nrows) return;
-    T *d = D + (size_t)rows[i] * K;
-    for (int k = threadIdx.x & 63; k < K; k += 64) d[k] = (T)0;
-}
-
 template <class TS, class T, int VEC, int Q, int LPE, bool MASK, bool DB, bool DV>
 __global__ __launch_bounds__(256) void k_bw_reside(BwArgs a) {
     const int blk = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
@@ -832,12 +828,25 @@ static int launch_bw(const BwArgs &a0, hipStream_t s) {
     return SWIFT_OK;
 }
 
+// After the walk's K slices: dB[row] += the carry slots of each row the
+// segments cut (split_fixup over all K; rows with more than FIX_LONG slots by
+// a workgroup each, the others by a wave each).
+template <class T>
+__global__ __launch_bounds__(256) void k_fix_rows(RowArgs a) {
+    if (blockIdx.x < (unsigned)a.nlong) {
+        split_fixup<T, 256>(a, (int)blockIdx.x);
+        return;
+    }
+    const int w = (int)(((blockIdx.x - a.nlong) * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (a.nlong + w < a.nfix) split_fixup<T, 64>(a, a.nlong + w);
+}
+
 // Any K: the slices of plan_row_slices (the forward walk's planner), each one
 // launch with its own lane shape; the full-vector shapes need K % VEC == 0
 // and 16-byte aligned operands, the rest of K takes masked scalar slices of
-// up to 64.  dB rows split by a segment boundary are zeroed once over all of
-// K; every slice stores its own columns of dB, and dvals takes the first
-// slice's partial dots, then each later slice's added on.
+// up to 64.  Every slice stores its own columns of dB (the pieces of cut rows
+// into carry slots, added by k_fix_rows after the last slice), and dvals
+// takes the first slice's partial dots, then each later slice's added on.
 template <class TS, class T, bool DB, bool DV, bool WALK>
 static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
     constexpr int V = V16<TS>::N;
@@ -845,9 +854,6 @@ static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
     BwArgs a = a0;
     const bool aligned = K % V == 0 &&
                          (((uintptr_t)a.G | (uintptr_t)a.H | (uintptr_t)a.dB) % 16) == 0;
-    if (DB && a.nsplit)
-        hipLaunchKernelGGL(k_zero_rows_bw<T>, dim3(ceil_div((size_t)a.nsplit * 64, 256)), dim3(256),
-                           0, s, a.split_rows, a.nsplit, (T *)a.dB, K);
     const auto slices = plan_row_slices(K, V, aligned);
     for (size_t i = 0; i < slices.size(); i++) {
         const RowTarget &t = slices[i].second;
@@ -874,6 +880,19 @@ static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
         set_error("no backward kernel for vec=%d q=%d lpe=%d", t.vec, t.q, t.lpe);
         return SWIFT_ERR_UNSUPPORTED;
     }
+    if (DB && a.nfix) {
+        RowArgs f{};
+        f.C = a.dB;
+        f.ldc = K;
+        f.ldk = K;
+        f.carry = a.carry;
+        f.fix = a.fix;
+        f.nfix = a.nfix;
+        f.nlong = a.nlong;
+        hipLaunchKernelGGL(k_fix_rows<T>, dim3((unsigned)a.nlong + ceil_div((size_t)(a.nfix - a.nlong) * 64, 256)),
+                           dim3(256), 0, s, f);
+        SWIFT_CHECK_LAUNCH();
+    }
     return SWIFT_OK;
 }
 
@@ -896,8 +915,10 @@ static BwArgs walk_args(const swift_matrix *W, int K) {
     a.rsrc = W->reside_src;
     a.rval = W->reside_val;
     a.perm = W->perm;
-    a.split_rows = W->prim.split_rows;
-    a.nsplit = W->prim.nsplit;
+    a.seg_slot = W->prim.seg_slot;
+    a.fix = W->prim.fix;
+    a.nfix = W->prim.nfix;
+    a.nlong = W->prim.nlong;
     a.K = K;
     return a;
 }
@@ -964,6 +985,8 @@ static int backward_t(const swift_matrix *AT, int K, const void *dC, const void 
         a.dvals = AT->dv_ws.ptr;
     }
     set_g_buffer<TS>(a, AT, K);
+    if (dB && AT->prim.nsplit)
+        SWIFT_TRY(carry_workspace(AT, (size_t)AT->prim.nsplit * K * sizeof(T), s, &a.carry));
     int rc;
     if (dB && dvals) rc = dispatch_bw<TS, T, true, true, true>(a, s);
     else if (dB) rc = dispatch_bw<TS, T, true, false, false>(a, s);

@@ -490,9 +490,9 @@ struct Workspace {
 static thread_local const Workspace *g_ws = nullptr;
 static thread_local size_t g_ws_off = 0;
 
-// Carry slots for a row-view product: from the caller's workspace, else the
-// matrix's workspace for this stream (grown on demand).
-static int carry_workspace(const swift_matrix *M, size_t bytes, hipStream_t s, void **out) {
+// Carry slots for a row-view product (or backward walk): from the caller's
+// workspace, else the matrix's workspace for this stream (grown on demand).
+int carry_workspace(const swift_matrix *M, size_t bytes, hipStream_t s, void **out) {
     if (g_ws) {
         const size_t off = (g_ws_off + 255) & ~(size_t)255;
         if (off + bytes > g_ws->bytes) {

@@ -102,6 +102,11 @@ int radix_sort_pairs(const unsigned *keys_in, const unsigned *vals_in,
 
 int bits_for(unsigned max_value);
 
+// Carry slots of bytes for a product or backward walk on M issued on stream
+// s: from swift_spmm_noalloc's scratch array when that call is in progress
+// on this thread, else from M's workspace for s (grown on demand).
+int carry_workspace(const swift_matrix *M, size_t bytes, hipStream_t s, void **out);
+
 // Fill helpers (kernels, never hipMemsetAsync, so that graph capture of the
 // SpMM stays valid).
 int fill_i32(int *p, int v, size_t n, hipStream_t s);
