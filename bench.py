@@ -4,14 +4,15 @@
 Workload: the 10^6 x 10^6, 16M-nonzero matrix (even rows banded with
 bandwidth 64 and 8..32 nonzeros, odd rows Zipf(1.8) lengths up to 20k with
 uniform columns, values N(0,1)), FP32, K = 128, row-major B (uniform in
-[-1,1], original column order) and C.  One step = one C = A*B (zeroing C, the
-regular tile kernel and the irregular reside kernel) with A's Swift format and
-B already resident in HBM.
+[-1,1], original column order) and C.  One step = one C = A*B (the row-view
+walk over the regular and reside entries, then the tail launch that adds the
+carry slots of rows cut by segment boundaries) with A's Swift format and B
+already resident in HBM.
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the regular
-tile kernel): algorithmic bytes per launch (CSR-equivalent A read once, B read
-once, C written once) over its average launch time, timed with HIP events on
-the launch stream.  `cpu_baseline` times the reference's own Swift_cpu
+Prints one JSON line.  `roofline` is for the dominant kernel (the row-view
+walk): algorithmic bytes per launch (CSR-equivalent A read once, B read once,
+C written once) over its average launch time, timed with HIP events on the
+launch stream.  `cpu_baseline` times the reference's own Swift_cpu
 (oracle/_ref, compiled from the reference sources) on the same matrix.
 """
 import argparse
@@ -305,7 +306,7 @@ def main():
     phases = np.array([M.spmm_timed(Bs, C, kernel=kern)
                        for _ in range(max(5, min(args.steps, 20)))])
     zero_ms, reg_ms, res_ms = [float(x) for x in np.median(phases, axis=0)]
-    log("phases (median ms): zero %.4f regular %.4f reside %.4f" % (zero_ms, reg_ms, res_ms))
+    log("phases (median ms): zero %.4f regular %.4f tail %.4f" % (zero_ms, reg_ms, res_ms))
     s_lp = 2 if bf16 else 4
     alg = algorithmic_bytes(m, n, nnz, K, s_a=s_lp, s_b=s_lp)
     achieved = alg / (reg_ms * 1e-3)
@@ -340,14 +341,17 @@ def main():
                    "build_ms": round(build_ms, 1),
                    "build_gpu_ms": dict(zip(("column_sort", "blocking", "schedules", "total"),
                                             (round(float(x), 2) for x in inf.build_ms))),
+                   # swift_spmm_timed's phases: zeroing (none for the row-view
+                   # walks), the walk, the tail (carry slots of cut rows, reside
+                   # blocks where the row view does not hold them)
                    "phase_ms": {"zero_C": round(zero_ms, 4), "regular": round(reg_ms, 4),
-                                "reside": round(res_ms, 4)}},
+                                "tail": round(res_ms, 4)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "traffic_frac": (round(traffic / (reg_ms * 1e-3) / HBM_PEAK, 4)
                                       if traffic else None),
-                     "kernel": "k_rowview_row (regular part, row view)",
+                     "kernel": "k_rowview_row (the row-view walk)",
                      "algorithmic_bytes": alg},
         "gpu": torch.cuda.get_device_name(0),
     }
