@@ -116,7 +116,10 @@ __global__ __launch_bounds__(256) void k_batch_walk(const RowArgs *__restrict__ 
                 case 128: batch_walk<32, BUF>(a, sid); break;
                 case 128 | (batch_alt_slots(128) << 12): batch_walk<32, BUF, batch_alt_slots(128)>(a, sid); break;
                 case 256 | (batch_alt_slots(256) << 12): batch_walk<64, BUF, batch_alt_slots(256)>(a, sid); break;
-                default: batch_walk<64, BUF>(a, sid); break;  // 256
+                // K = 256 at 8 slot rows: the rule's 16 for a single product
+                // would raise the register count of the kernel that holds
+                // every shape
+                default: batch_walk<64, BUF, 8>(a, sid); break;  // 256
             }
         }
     }

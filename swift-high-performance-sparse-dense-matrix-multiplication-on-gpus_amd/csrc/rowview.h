@@ -693,6 +693,10 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 #ifndef SWIFT_UNROLL_LONG
 #define SWIFT_UNROLL_LONG 8
 #endif
+// One-chunk 64-lane groups of 4- and 8-byte elements (FP32 K=256, FP64
+// K=128) take 16 slots: on the 1M-row matrix FP32 K=256 1.550 -> 1.519 ms,
+// FP64 K=128 1.531 -> 1.523 ms; BF16 K=256 was slower (0.896 -> 0.904 ms)
+// and keeps 8.
 
 //
 // WIN: the workgroup has staged a window of B rows in LDS (win).  A step
@@ -707,7 +711,8 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 // SLOTS: slot rows (B-row loads in flight per lane group) per step; 0 takes
 // rule_slots, other values come from the autotuner.
 constexpr int rule_slots(int lpe, int q, int elem, bool win = false) {
-    return (lpe >= 32 && q <= 2 && !win) ? SWIFT_UNROLL_LONG
+    return (lpe == 64 && q == 1 && elem >= 4 && !win) ? 16
+           : (lpe >= 32 && q <= 2 && !win) ? SWIFT_UNROLL_LONG
            : elem == 2                   ? SWIFT_UNROLL_NARROW
                                          : SWIFT_UNROLL_WIDE;
 }

@@ -751,7 +751,11 @@ __global__ __launch_bounds__(256) void k_transpose16(const T *__restrict__ in, T
         const int rr = (t >> 5) + 8 * i, cv = t & 31;
         const int r = r0 + rr, c = c0 + cv * V;
         if (r < rows && c < cols) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(in + (size_t)r * cols + c);
+            // read once and written once: non-temporal loads and stores (a
+            // C2 column-major product 1.115 -> 1.110 ms)
+            const u32x4 vv =
+                __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(in + (size_t)r * cols + c));
+            const uint4 v = make_uint4(vv.x, vv.y, vv.z, vv.w);
             const T *e = reinterpret_cast<const T *>(&v);
 #pragma unroll
             for (int k = 0; k < V; k++) tile[rr][cv * V + k] = e[k];
@@ -776,9 +780,9 @@ __global__ __launch_bounds__(256) void k_transpose16(const T *__restrict__ in, T
 #pragma unroll
                 for (int k = 0; k < V; k++) x[k] = (T)(x[k] + e[k]);
             }
-            uint4 w;
+            u32x4 w;
             __builtin_memcpy(&w, x, 16);
-            *reinterpret_cast<uint4 *>(o) = w;
+            __builtin_nontemporal_store(w, reinterpret_cast<u32x4 *>(o));
         }
     }
 }
