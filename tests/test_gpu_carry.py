@@ -168,3 +168,30 @@ def test_cut_rows_repeatable(swift, oracle, cuda, name):
     b = _product(swift, M, Bt, 128, swift.KERNEL_DEFAULT, fill=0.0)
     assert np.array_equal(a, b)
     M.close()
+
+
+def test_cut_rows_two_streams(swift, oracle, cuda):
+    """Products of one handle on two streams at once: each stream has carry
+    slots of its own, so neither overwrites the other's pieces."""
+    import torch
+    m, n, rp, ci, v = CASES["powerlaw_3000x2500"]
+    vv = np.asarray(v, np.float32)
+    rng = np.random.default_rng(9)
+    Bs = [rng.uniform(-1, 1, (n, 128)).astype(np.float32) for _ in range(2)]
+    refs = [oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), b.astype(np.float64)) for b in Bs]
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=torch.float32)
+    M.set_segment_items(8)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    Bt = [torch.tensor(b, device="cuda") for b in Bs]
+    outs = [torch.full((m, 128), float("nan"), device="cuda") for _ in range(2)]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for s, b, o in zip(streams, Bt, outs):
+            with torch.cuda.stream(s):
+                M.spmm(b, out=o, stream=s)
+        torch.cuda.synchronize()
+        for o, (C64, absC) in zip(outs, refs):
+            got = o.cpu().numpy()
+            assert not np.isnan(got).any()
+            assert gpu_common.rel_err(got, C64, absC) <= 1e-5, rep
+    M.close()
