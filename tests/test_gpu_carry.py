@@ -4,9 +4,10 @@ A row-view walk stores the piece of a cut row that holds the row's start into
 C and every later piece into a carry slot; one tail launch adds each cut
 row's slots onto it in slot order (one wave per row, a workgroup for rows of
 more than 32 slots) while its other workgroups run the reside blocks with
-atomics.  Small segment sizes (W = 8 cuts almost every row, the 400-entry
-rows of powerlaw_3000x2500 into about 50 pieces) put every matrix through
-that path; C starts as NaN so a row left unwritten shows up.  Checked against
+atomics.  Small segment sizes (W = 8 cuts almost every row) put every
+matrix through that path, and a matrix with rows of 600-900 entries gives
+rows of more than 32 slots; C starts as NaN so a row left unwritten shows
+up.  Checked against
 the FP64 oracle for FP32 (plain and accumulating), FP64 and BF16, for the
 default, row-view and window walks, at K with one lane shape and with masked
 slices (K = 100); the split count of the default table against a NumPy
@@ -20,6 +21,22 @@ import gpu_common
 pytestmark = pytest.mark.gpu
 
 CASES = gpu_common.cases()
+
+
+def long_rows(m=600, n=1000, seed=17):
+    """Ten rows of 600-900 entries among short random rows: at W = 8 the
+    long rows are cut into 75-113 pieces."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(m):
+        k = int(rng.integers(600, 901)) if i % 60 == 7 else int(rng.integers(0, 12))
+        rows.append(np.sort(rng.choice(n, size=k, replace=False)))
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    ci = np.concatenate(rows).astype(np.int32)
+    return m, n, rp, ci, rng.standard_normal(len(ci))
+
+
+CASES["longrows_600x1000"] = long_rows()
 
 
 def _product(swift, M, B, K, kernel, flags=0, dtype=None, fill=float("nan"), C0=None):
@@ -54,7 +71,7 @@ def test_cut_rows_f32(swift, oracle, cuda, name, W, K, kernel):
 
 @pytest.mark.parametrize("K", [32, 128])
 @pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "tall_2000x64",
-                                  "banded_mixed_4000"])
+                                  "banded_mixed_4000", "longrows_600x1000"])
 def test_cut_rows_accumulate(swift, oracle, cuda, name, K):
     """C += A*B: the piece holding a row's start adds onto C, the fix-up adds
     the rest."""
@@ -73,7 +90,8 @@ def test_cut_rows_accumulate(swift, oracle, cuda, name, K):
     M.close()
 
 
-@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "random_300x200"])
+@pytest.mark.parametrize("name", ["powerlaw_3000x2500", "densecol_400x100", "random_300x200",
+                                  "longrows_600x1000"])
 def test_cut_rows_f64_bf16(swift, oracle, cuda, name):
     import torch
     m, n, rp, ci, v = CASES[name]
@@ -194,4 +212,24 @@ def test_cut_rows_two_streams(swift, oracle, cuda):
             got = o.cpu().numpy()
             assert not np.isnan(got).any()
             assert gpu_common.rel_err(got, C64, absC) <= 1e-5, rep
+    M.close()
+
+
+@pytest.mark.parametrize("K", [33, 1100])
+@pytest.mark.parametrize("name", ["longrows_600x1000", "densecol_400x100"])
+def test_cut_rows_fixup_shapes(swift, oracle, cuda, name, K):
+    """The fix-up's other shapes: K = 33 (no 16-byte vectors: one element
+    per thread, a masked K slice) and K = 1100 (more than 256 vectors per
+    row: the workgroup loops over the row with one group of threads), on rows
+    of more than 32 slots (workgroup path) and fewer (wave path)."""
+    import torch
+    m, n, rp, ci, v = CASES[name]
+    vv = np.asarray(v, np.float32)
+    B = np.random.default_rng(K).uniform(-1, 1, (n, K)).astype(np.float32)
+    C64, absC = oracle.spmm_f64(m, n, rp, ci, vv.astype(np.float64), B.astype(np.float64))
+    M = swift.SwiftMatrix(m, n, rp, ci, vv, dtype=torch.float32)
+    M.set_segment_items(8)
+    C = _product(swift, M, torch.tensor(B, device="cuda"), K, swift.KERNEL_DEFAULT)
+    assert not np.isnan(C).any()
+    assert gpu_common.rel_err(C, C64, absC) <= 1e-5
     M.close()
