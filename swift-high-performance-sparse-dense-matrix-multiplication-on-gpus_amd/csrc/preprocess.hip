@@ -718,7 +718,7 @@ static void free_matrix_arrays(swift_matrix *M) {
                     M->blkstart, M->blkstop, M->blkptr, M->val_position, M->val_offset,
                     M->rowsorted, M->rowsort_src, M->csr_rowptr, M->csr_colidx,
                     M->sortval_lp, M->reside_val_lp, M->work, M->rv_ptr, M->rv_col,
-                    M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->rv_row, M->empty_rows, M->resrow_flag, M->reside_rows, M->dv_rank,
+                    M->rv_val, M->rv_src, M->rv_val_lp, M->inv_perm, M->rv_row, M->empty_rows, M->resrow_flag, M->dv_rank,
                     M->dv_off, M->dv_dst};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1183,7 +1183,7 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
             SWIFT_CHECK_LAUNCH();
         }
         M->nempty = ne;
-        // rows that receive reside entries (flags and list)
+        // rows that receive reside entries
         SWIFT_TRY(fill_i32(flag, 0, (size_t)m + 1, s));
         if (M->reside_nnz) {
             hipLaunchKernelGGL(k_mark_rows, dim3(grid1(M->reside_nnz)), dim3(256), 0, s,
@@ -1194,17 +1194,6 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
         hipLaunchKernelGGL(k_flag_bytes, dim3(grid1(m)), dim3(256), 0, s, m, flag,
                            M->resrow_flag);
         SWIFT_CHECK_LAUNCH();
-        SWIFT_TRY(exclusive_scan(flag, idx.as<int>(), (size_t)m, idx.as<int>() + m, tmp.ptr, s));
-        int nr = 0;
-        SWIFT_HIP(hipMemcpyAsync(&nr, idx.as<int>() + m, sizeof(int), hipMemcpyDeviceToHost, s));
-        SWIFT_HIP(hipStreamSynchronize(s));
-        SWIFT_TRY(dalloc(&M->reside_rows, (size_t)nr));
-        if (nr) {
-            hipLaunchKernelGGL(k_empty_compact, dim3(grid1(m)), dim3(256), 0, s, m, flag,
-                               idx.as<int>(), M->reside_rows);
-            SWIFT_CHECK_LAUNCH();
-        }
-        M->nreside_rows = nr;
     }
     SWIFT_HIP(hipStreamSynchronize(s));
     SWIFT_TRY(build_segments(M, 128, s));

@@ -205,15 +205,12 @@ struct swift_matrix {
     // transposed layout of the backward pass: its walks then need no reside
     // launch, whose atomics into dB cost more than walking the few entries.
     // build_rowview also sets it for a matrix whose reside part is at most 1%
-    // of its entries and reaches more than 1 row in 64.
+    // of its entries.
     bool rv_all = false;
     int rv_nnz = 0;
-    // rows that receive reside entries: the default walk stores them with
-    // atomics after zeroing them with the split rows, so the reside blocks can
-    // run inside the walk's launch
+    // rows that receive reside entries: the fix-up of a cut row adds with
+    // atomics where reside blocks of the same tail launch add into the row
     unsigned char *resrow_flag = nullptr;  // m bytes, 1 = row has reside entries
-    int *reside_rows = nullptr;            // those rows, ascending
-    int nreside_rows = 0;
     // Merge-path split of (rows + regular entries) into segments of W items.
     // A boundary that falls inside a row cuts it into pieces: the piece
     // holding the row's start is stored into C like any row, every later
