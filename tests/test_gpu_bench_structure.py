@@ -25,8 +25,8 @@ def matrix(swift, oracle):
 
 def test_matrix_structure(swift, oracle, cuda):
     """Rows longer than the default segments (the row view holds 17 items per
-    row on average, so segments of 96 or more) and a reside part of under 1%
-    of the entries."""
+    row on average and a default segment at least three such rows, 51 items)
+    and a reside part of under 1% of the entries."""
     import torch
     d, rows = matrix(swift, oracle)
     lens = np.diff(d["rowptr"])

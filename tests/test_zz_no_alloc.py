@@ -1,8 +1,10 @@
 """swift_spmm_noalloc: swift_spmm without device allocation.
 
-The column-major default product transposes B and C through a workspace;
-swift_spmm grows one per matrix handle with hipMalloc, swift_spmm_noalloc
-takes it from the library's per-GPU scratch array.  Checked through the C ABI:
+The column-major default product transposes B and C through a workspace, and
+the row-view walks keep the partial sums of rows that segment boundaries cut
+in carry slots; swift_spmm grows these per matrix handle (the carry slots per
+stream) with hipMalloc, swift_spmm_noalloc takes them from the library's
+per-GPU scratch array.  Checked through the C ABI:
 two calls in a row (the second on a smaller input, so stale scratch contents
 from the first would show) against swift_spmm's output and against the FP64
 oracle at the FP32 / FP64 bounds; row-major products; the size limit; and that
@@ -69,7 +71,7 @@ def test_noalloc_matches_swift_spmm(swift, oracle, cuda, layout):
         assert not np.isnan(got).any()
         assert gpu_common.rel_err(got, C64, absC) <= tol
         assert gpu_common.rel_err(ref, C64, absC) <= tol
-        # split rows add with atomics in either call, in any order
+        # cut rows that also take reside entries add with atomics, in any order
         assert gpu_common.rel_err(got, ref.astype(np.float64), absC) <= 2 * tol
         M.close()
 
