@@ -567,9 +567,13 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
     const int2 st = a.seg[sid], en = a.seg[sid + 1];
     const int r1 = en.x, e1 = en.y;
     int row = st.x, e = st.y;
+    // requested at once, as in the forward walk: the carry slot of the
+    // boundary before the segment and whether its end cuts the last row
+    const int slot_before = DB && sid > 0 ? a.seg_slot[sid - 1] : 0;
     const bool carried_in = row < a.m && e > a.rv_ptr[row];
+    const bool cut_end = r1 < a.m && e1 > a.rv_ptr[r1];
     // carry slot of the dB piece this segment starts with
-    const int cslot = DB && carried_in ? a.seg_slot[sid - 1] : 0;
+    const int cslot = carried_in ? slot_before : 0;
     const TS *val = static_cast<const TS *>(a.rv_val);
     const TS *G = static_cast<const TS *>(a.G) + a.kbase;
     const TS *H = static_cast<const TS *>(a.H) + a.kbase;
@@ -698,7 +702,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
             row++;
             continue;
         }
-        if (last && !(r1 < a.m && e1 > a.rv_ptr[r1])) break;
+        if (last && !cut_end) break;
 #pragma unroll
         for (int o = LPE; o < 64; o <<= 1)
 #pragma unroll

@@ -775,9 +775,15 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
     const int2 st = a.seg[sid], en = a.seg[sid + 1];
     const int r1 = en.x, e1 = en.y;
     int row = st.x, e = st.y;
+    // Everything the segment's first and last row need is requested at once,
+    // beside the staging below: the carry slot of the boundary before it
+    // (read whether or not that boundary cuts a row) and whether the segment
+    // end cuts its last row.
+    const int slot_before = CARRY && sid > 0 ? a.seg_slot[sid - 1] : 0;
     const bool carried_in = row < a.m && e > a.rv_ptr[row];
-    // carry slot of the piece this segment starts with (the boundary before it)
-    const int cslot = CARRY && carried_in ? a.seg_slot[sid - 1] : 0;
+    const bool cut_end = r1 < a.m && e1 > a.rv_ptr[r1];
+    // carry slot of the piece this segment starts with
+    const int cslot = carried_in ? slot_before : 0;
     const TV *val = static_cast<const TV *>(a.rv_val);
     const TB *B = static_cast<const TB *>(a.B);
     __amdgpu_buffer_rsrc_t rsrc;
@@ -935,7 +941,7 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
             e += nb;
         }
         const bool last = row >= r1;
-        if (last && !(r1 < a.m && e1 > a.rv_ptr[r1])) break;
+        if (last && !cut_end) break;
         // combine the EPS sub-groups' partial sums of this row
 #pragma unroll
         for (int o = LPE; o < 64; o <<= 1)
