@@ -209,8 +209,9 @@ int swift_row_idx_sort(swift_matrix *mat, swift_stream_t stream);
 
 /* Schedule knob of the row-view kernels: items (rows + entries) per
  * merge-path segment, i.e. per wave.  By default the size follows the bytes
- * of one B row (48 items for rows of 1 KiB or more, 64 from 512 B, 128 from
- * 256 B, 256 below), and is at least three mean rows of the matrix; a call
+ * of one B row (48 items for rows of 1 KiB or more, 64 from 512 B, from
+ * 256 B 128 for 16-bit elements and 96 for others, 128 below), and is at
+ * least three mean rows of the matrix; a call
  * here pins one size for every product of
  * the matrix.  Rebuilds the segment table on the GPU; the format
  * itself is unchanged.  The batch autotuner (swift_batch_create) and

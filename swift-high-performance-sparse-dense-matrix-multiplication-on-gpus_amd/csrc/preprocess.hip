@@ -1087,7 +1087,12 @@ int build_segments(swift_matrix *M, int W, hipStream_t s) {
 // mean rows (items per row), which a matrix of long rows (the 200k-node
 // graph: 51 items per row) prefers.  A cut row costs one carry slot.
 int build_class_segments(swift_matrix *M, hipStream_t s) {
-    static const int by_bytes[SEG_CLASSES] = {48, 64, 128, 128, 256};
+    // measured on the 1M-row matrix with the carry slots and the batch
+    // prefetch: FP32 K=32 128 items 0.258 ms against 256 items 0.270 ms, FP64
+    // K=32 96 items 0.348 against 128 items 0.361 ms, FP32 K=64 96 items
+    // 0.340 against 128 items 0.352 ms, BF16 K=32 128 items 0.264 against
+    // 256 items 0.268 ms
+    static const int by_bytes[SEG_CLASSES] = {48, 64, 128, 96, 128};
     const long long m = std::max(M->m, 1);
     const int mean = (int)((m + M->rv_nnz + m - 1) / m);
     for (int c = 0; c < SEG_CLASSES; c++) {
