@@ -208,8 +208,10 @@ def vendor_baseline(scale, K, nnz):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # 100 steps of ~0.7 ms: long enough that the GPU's clock ramp after the
+    # format build (the first ~20 products run 2-3% slower) weighs little
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the matrix (tests only)")
     ap.add_argument("--K", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
