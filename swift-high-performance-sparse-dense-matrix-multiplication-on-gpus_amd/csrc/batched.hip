@@ -59,7 +59,7 @@ __global__ void k_batch_reset(int *counter) {
 
 template <int LPE, bool BUF, int SLOTS = 0>
 __device__ __forceinline__ void batch_walk(const RowArgs &a, int sid) {
-    rowview_walk<float, float, float, 4, 1, LPE, false, BUF, false, true, SLOTS>(a, sid);
+    rowview_walk<float, float, float, 4, 1, LPE, false, BUF, false, SLOTS>(a, sid);
 }
 
 // Slot-row count the batched walk can run besides the rule for a K: 8 for

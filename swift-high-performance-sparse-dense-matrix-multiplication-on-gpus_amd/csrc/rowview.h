@@ -245,7 +245,7 @@ struct RowArgs {
     // buffer-resource path (row-view walk): bytes of B from its K-slice start,
     // and the row stride in bytes
     unsigned bbytes, bstride;
-    // row-view walk with carry slots (rowview_walk CARRY): a segment that
+    // row-view walk's carry slots: a segment that
     // starts inside a row stores that row's piece into carry slot
     // seg_slot[sid - 1] (ldk values per slot, the K slice at kbase).  The
     // fix-up (split_fixup) adds the nfix cut rows' slots onto C: fix[i] =
@@ -677,7 +677,7 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 // entries h, h+EPS, ... with UNROLL 16-byte loads per lane in flight; at a
 // row end the sub-groups' sums are combined with a butterfly and sub-group 0
 // stores the row.  Control flow is uniform across the wave.  Rows cut by a
-// segment boundary are split rows, stored in pieces (CARRY below).
+// segment boundary are split rows, stored in pieces (carry slots below).
 #ifndef SWIFT_UNROLL_NARROW
 #define SWIFT_UNROLL_NARROW 4
 #endif
@@ -703,10 +703,10 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 // whose used slots all fall inside the window reads them from LDS; any other
 // step reads B from memory as without a window.
 //
-// CARRY: a segment that starts inside a row stores that row's piece into the
-// row's carry slot for the fix-up pass (split_fixup); every other row,
-// including the first piece of a row the segment end cuts, is stored into C
-// once.  Without CARRY both pieces take atomics onto a row zeroed beforehand.
+// Carry slots: a segment that starts inside a row stores that row's piece
+// into the carry slot of the boundary it starts at, for the fix-up pass
+// (split_fixup); every other row, including the first piece of a row the
+// segment end cuts, is stored into C once.
 //
 // SLOTS: slot rows (B-row loads in flight per lane group) per step; 0 takes
 // rule_slots, other values come from the autotuner.
@@ -758,7 +758,7 @@ inline std::vector<int> slot_candidates(int K, size_t elem, bool aligned) {
 }
 
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK, bool BUF,
-          bool WIN = false, bool CARRY = false, int SLOTS = 0>
+          bool WIN = false, int SLOTS = 0>
 __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window win = Window{}) {
     static_assert(!WIN || (BUF && !MASK), "the window walk is a buffer-load, unmasked walk");
     constexpr int EPS = 64 / LPE;
@@ -779,7 +779,7 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
     // beside the staging below: the carry slot of the boundary before it
     // (read whether or not that boundary cuts a row) and whether the segment
     // end cuts its last row.
-    const int slot_before = CARRY && sid > 0 ? a.seg_slot[sid - 1] : 0;
+    const int slot_before = sid > 0 ? a.seg_slot[sid - 1] : 0;
     const bool carried_in = row < a.m && e > a.rv_ptr[row];
     const bool cut_end = r1 < a.m && e1 > a.rv_ptr[r1];
     // carry slot of the piece this segment starts with
@@ -950,19 +950,12 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
 #pragma unroll
                 for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
         if (h == 0) {
-            if constexpr (CARRY) {
-                if (row == st.x && carried_in)
-                    put_carry<TC, VEC, Q, LPE, MASK, HOIST>(a, cslot, sub, acc,
-                                                           HOIST ? osc : nullptr);
-                else
-                    put_row<TC, VEC, Q, LPE, MASK, HOIST>(a, row, sub, false, acc,
-                                                         HOIST ? osc : nullptr);
-            } else {
-                // rows that other waves also add into take atomics
-                const bool shared = last || (row == st.x && carried_in);
-                put_row<TC, VEC, Q, LPE, MASK, HOIST>(a, row, sub, shared, acc,
+            if (row == st.x && carried_in)
+                put_carry<TC, VEC, Q, LPE, MASK, HOIST>(a, cslot, sub, acc,
+                                                       HOIST ? osc : nullptr);
+            else
+                put_row<TC, VEC, Q, LPE, MASK, HOIST>(a, row, sub, false, acc,
                                                      HOIST ? osc : nullptr);
-            }
         } else
 #pragma unroll
             for (int q = 0; q < Q; q++)

@@ -102,7 +102,7 @@ void k_rowview_row(RowArgs a) {
     // wave-uniform segment id, kept in a scalar register
     const int sid = __builtin_amdgcn_readfirstlane((int)((lb * (size_t)blockDim.x + threadIdx.x) >> 6));
     if (sid >= a.nseg) return;
-    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF, false, true, SLOTS>(a, sid);
+    rowview_walk<TV, TB, TC, VEC, Q, LPE, MASK, BUF, false, SLOTS>(a, sid);
 }
 
 // The row-view walk with a.slots slot rows per step where the autotuner
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(WIN_THREADS, 8) void k_rowview_win(RowArgs a) {
     }
     // wave-uniform segment id, kept in a scalar register
     const int sid = __builtin_amdgcn_readfirstlane(s0 + wave);
-    if (sid < a.nseg) rowview_walk<TV, TB, TC, VEC, Q, LPE, false, true, true, true>(a, sid, win);
+    if (sid < a.nseg) rowview_walk<TV, TB, TC, VEC, Q, LPE, false, true, true>(a, sid, win);
 }
 
 template <class TV, class TB, class TC, int VEC, int Q, int LPE, bool MASK>
