@@ -255,8 +255,8 @@ int build_transpose_impl(const swift_matrix *A, swift_matrix **out, hipStream_t 
 //   fused over A^T:  row j of A^T (= column j of A) keeps H = B[j]; each entry
 //                    (i, v) at row-view position p gathers G = dC[i] once and
 //                    uses it twice: dv[p] = <H, G> and dB[j] += v * G; dv is
-//                    then reordered into A's CSR order (k_dv_to_buckets,
-//                    k_dv_bucket_order).
+//                    then reordered into A's CSR order (pass 1 in
+//                    k_bw_tail, pass 2 in k_dv_bucket_order).
 // The walked matrix's reside blocks (one column, <= 32 entries) follow with
 // the roles swapped: the column operand is loaded once, dB takes atomics.
 
@@ -744,13 +744,13 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
 // in every bucket; neighbouring chunks run on one XCD, so the lines the two
 // runs share are completed in one L2.
 template <class T, int LG>
-__global__ __launch_bounds__(1024) void k_dv_to_buckets(const T *__restrict__ v,
-                                                        const unsigned short *__restrict__ rank,
-                                                        const int *__restrict__ dst, size_t n,
-                                                        T *__restrict__ out) {
+__device__ __forceinline__ void dv_chunk_to_buckets(const T *__restrict__ v,
+                                                    const unsigned short *__restrict__ rank,
+                                                    const int *__restrict__ dst, size_t n,
+                                                    T *__restrict__ out, unsigned chunk) {
     constexpr int PER = (1 << LG) / 1024;
     __shared__ T lds[1 << LG];
-    const size_t base = (size_t)xcd_block_bw(blockIdx.x, gridDim.x) << LG;
+    const size_t base = (size_t)chunk << LG;
     const int cnt = (int)min((size_t)1 << LG, n - base);
     unsigned short r[PER];
     T x[PER];
@@ -777,6 +777,33 @@ __global__ __launch_bounds__(1024) void k_dv_to_buckets(const T *__restrict__ v,
         const int t = threadIdx.x + i * 1024;
         if (t < cnt) out[d[i]] = lds[t];
     }
+}
+
+// The fused backward's tail: the fix-up of the dB rows that segment
+// boundaries cut and pass 1 of the dvals reorder, which both wait only on
+// the walk, in one launch.  Workgroups [0, long_blocks) fix up the rows of
+// more than FIX_LONG slots, four at a time (one per 256-thread group); then
+// up to fix_blocks the other cut rows, one per wave; the rest are pass 1's
+// chunks.  On the 200k-node graph the separate fix-up launch took 9 us; the
+// fused backward went 0.617 -> 0.614 ms (K=128) and 0.205 -> 0.202 ms (K=32).
+template <class T, int LG>
+__global__ __launch_bounds__(1024) void k_bw_tail(RowArgs f, unsigned long_blocks,
+                                                  unsigned fix_blocks, const T *__restrict__ v,
+                                                  const unsigned short *__restrict__ rank,
+                                                  const int *__restrict__ dst, size_t n,
+                                                  T *__restrict__ out) {
+    if (blockIdx.x < long_blocks) {  // uniform over the workgroup
+        const int i = (int)(blockIdx.x * 4 + threadIdx.x / 256);
+        split_fixup<T, 256, 4>(f, i, i < f.nlong);
+        return;
+    }
+    if (blockIdx.x < fix_blocks) {
+        const int w = (int)(((blockIdx.x - long_blocks) * (size_t)blockDim.x + threadIdx.x) >> 6);
+        if (f.nlong + w < f.nfix) split_fixup<T, 64>(f, f.nlong + w);
+        return;
+    }
+    dv_chunk_to_buckets<T, LG>(v, rank, dst, n, out,
+                               xcd_block_bw(blockIdx.x - fix_blocks, gridDim.x - fix_blocks));
 }
 
 // Pass 2, in place: one workgroup per bucket of 2^LG CSR positions; its
@@ -863,8 +890,22 @@ __global__ __launch_bounds__(256) void k_fix_rows(RowArgs a) {
 // up to 64.  Every slice stores its own columns of dB (the pieces of cut rows
 // into carry slots, added by k_fix_rows after the last slice), and dvals
 // takes the first slice's partial dots, then each later slice's added on.
+// RowArgs of the fix-up of dB's cut rows (split_fixup).
+static RowArgs fix_args(const BwArgs &a) {
+    RowArgs f{};
+    f.C = a.dB;
+    f.ldc = a.K;
+    f.ldk = a.K;
+    f.carry = a.carry;
+    f.fix = a.fix;
+    f.nfix = a.nfix;
+    f.nlong = a.nlong;
+    return f;
+}
+
+// defer_fix: the caller runs the fix-up of dB's cut rows itself (k_bw_tail).
 template <class TS, class T, bool DB, bool DV, bool WALK>
-static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
+static int dispatch_bw(const BwArgs &a0, hipStream_t s, bool defer_fix = false) {
     constexpr int V = V16<TS>::N;
     const int K = a0.K;
     BwArgs a = a0;
@@ -896,17 +937,9 @@ static int dispatch_bw(const BwArgs &a0, hipStream_t s) {
         set_error("no backward kernel for vec=%d q=%d lpe=%d", t.vec, t.q, t.lpe);
         return SWIFT_ERR_UNSUPPORTED;
     }
-    if (DB && a.nfix) {
-        RowArgs f{};
-        f.C = a.dB;
-        f.ldc = K;
-        f.ldk = K;
-        f.carry = a.carry;
-        f.fix = a.fix;
-        f.nfix = a.nfix;
-        f.nlong = a.nlong;
+    if (DB && a.nfix && !defer_fix) {
         hipLaunchKernelGGL(k_fix_rows<T>, dim3((unsigned)a.nlong + ceil_div((size_t)(a.nfix - a.nlong) * 64, 256)),
-                           dim3(256), 0, s, f);
+                           dim3(256), 0, s, fix_args(a));
         SWIFT_CHECK_LAUNCH();
     }
     return SWIFT_OK;
@@ -1003,16 +1036,24 @@ static int backward_t(const swift_matrix *AT, int K, const void *dC, const void 
     set_g_buffer<TS>(a, AT, K);
     if (dB && AT->prim.nsplit)
         SWIFT_TRY(carry_workspace(AT, (size_t)AT->prim.nsplit * K * sizeof(T), s, &a.carry));
+    // with dB and walk-order dvals, the fix-up of dB's cut rows runs in the
+    // launch of the reorder's first pass (k_bw_tail)
+    const bool fused_tail = dB && walk;
     int rc;
-    if (dB && dvals) rc = dispatch_bw<TS, T, true, true, true>(a, s);
+    if (dB && dvals) rc = dispatch_bw<TS, T, true, true, true>(a, s, fused_tail);
     else if (dB) rc = dispatch_bw<TS, T, true, false, false>(a, s);
     else rc = dispatch_bw<TS, T, false, true, true>(a, s);
     if (rc != SWIFT_OK || !walk) return rc;
     const size_t n = (size_t)AT->nnz;
     using P = DvPlan<T>;
-    hipLaunchKernelGGL((k_dv_to_buckets<T, P::CHUNK>), dim3(ceil_div(n, (size_t)1 << P::CHUNK)),
-                       dim3(1024), 0, s, (const T *)AT->dv_ws.ptr, AT->dv_rank, AT->dv_dst, n,
-                       (T *)dvals);
+    const unsigned chunks = ceil_div(n, (size_t)1 << P::CHUNK);
+    const unsigned nfix = fused_tail ? (unsigned)a.nfix : 0u;
+    const unsigned nlong = fused_tail ? (unsigned)a.nlong : 0u;
+    const unsigned long_blocks = ceil_div(nlong, 4);
+    const unsigned fix_blocks = long_blocks + ceil_div((size_t)(nfix - nlong) * 64, 1024);
+    hipLaunchKernelGGL((k_bw_tail<T, P::CHUNK>), dim3(fix_blocks + chunks), dim3(1024), 0, s,
+                       fix_args(a), long_blocks, fix_blocks, (const T *)AT->dv_ws.ptr,
+                       AT->dv_rank, AT->dv_dst, n, (T *)dvals);
     hipLaunchKernelGGL((k_dv_bucket_order<T, P::BUCKET>), dim3(ceil_div(n, (size_t)1 << P::BUCKET)),
                        dim3(1024), 0, s, (T *)dvals, AT->dv_off, n);
     SWIFT_CHECK_LAUNCH();

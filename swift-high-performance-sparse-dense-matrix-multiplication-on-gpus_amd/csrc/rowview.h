@@ -451,17 +451,22 @@ __device__ __forceinline__ void put_carry(const RowArgs &a, int slot, int sub,
 // are added in group order (shuffles inside a wave, LDS across a
 // workgroup).  The row's values (its first piece) are read while the slots
 // load.  Rows that reside blocks of the same launch add into take atomics.
-template <class TC, int NT>
-__device__ __forceinline__ void split_fixup(const RowArgs &a, int i) {
-    static_assert(NT == 64 || NT == 256, "one wave or one 256-thread workgroup");
+// GROUPS 256-thread groups of one workgroup may fix up GROUPS rows at once:
+// group g is threads [256 g, 256 g + 256) and has its own LDS share; a group
+// without a row (active false) still takes part in every barrier.
+template <class TC, int NT, int GROUPS = 1>
+__device__ __forceinline__ void split_fixup(const RowArgs &a, int i, bool active = true) {
+    static_assert(NT == 64 || NT == 256, "one wave or 256-thread workgroup groups");
+    static_assert(GROUPS == 1 || NT == 256, "several groups need 256-thread groups");
     constexpr int V16 = 16 / (int)sizeof(TC);
     constexpr int U = 8;  // slots in flight per thread
-    const int t = NT == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
-    const int4 f = a.fix[i];
+    const int t = NT == 64 ? (int)(threadIdx.x & 63) : (int)(threadIdx.x & (NT - 1));
+    const int g = GROUPS == 1 ? 0 : (int)(threadIdx.x / NT);
+    const int4 f = active ? a.fix[i] : make_int4(0, 0, 0, 0);
     const int row = f.x, b0 = f.y, b1 = f.z;
     TC *crow = static_cast<TC *>(a.C) + (size_t)row * a.ldc;
     const TC *carry = static_cast<const TC *>(a.carry);
-    const bool atom = a.rflag && a.rflag[row];
+    const bool atom = active && a.rflag && a.rflag[row];
     const int K = a.ldk;
     const bool vec = (K % V16) == 0 && (a.ldc % V16) == 0 && ((uintptr_t)a.C % 16) == 0 &&
                      ((uintptr_t)a.carry % 16) == 0;
@@ -472,7 +477,7 @@ __device__ __forceinline__ void split_fixup(const RowArgs &a, int i) {
     const int p = t / TE, j0 = t % TE;
     for (int jb = 0; jb < T; jb += TE) {
         const int j = jb + j0;
-        const bool mine = p < P && j < T;
+        const bool mine = active && p < P && j < T;
         TC cur[V16], acc[V16];
 #pragma unroll
         for (int v = 0; v < V16; v++) cur[v] = acc[v] = (TC)0;
@@ -515,15 +520,16 @@ __device__ __forceinline__ void split_fixup(const RowArgs &a, int i) {
                     if (p == 0) acc[v] += y;
                 }
         } else {
-            __shared__ TC part[NT * V16];
+            __shared__ TC part[GROUPS * NT * V16];
+            TC *gp = part + g * NT * V16;
             if (P > 1) {
 #pragma unroll
-                for (int v = 0; v < V16; v++) part[t * V16 + v] = acc[v];
+                for (int v = 0; v < V16; v++) gp[t * V16 + v] = acc[v];
                 __syncthreads();
                 if (p == 0)
                     for (int q = 1; q < P; q++)
 #pragma unroll
-                        for (int v = 0; v < V16; v++) acc[v] += part[(q * TE + j0) * V16 + v];
+                        for (int v = 0; v < V16; v++) acc[v] += gp[(q * TE + j0) * V16 + v];
                 __syncthreads();
             }
         }
