@@ -703,9 +703,25 @@ __global__ __launch_bounds__(256) void k_row_sort(int tilenum, const int *__rest
 // ------------------------------------------------------------ matrix build --
 
 static void free_seg_table(swift_matrix::SegTable &t) {
-    for (void *p : {(void *)t.seg, (void *)t.split_rows, (void *)t.seg_slot, (void *)t.fix})
-        if (p) (void)hipFree(p);
+    if (!t.shared)
+        for (void *p : {(void *)t.seg, (void *)t.split_rows, (void *)t.seg_slot, (void *)t.fix})
+            if (p) (void)hipFree(p);
     t = swift_matrix::SegTable();
+}
+
+// A table of W items that one of the matrix's default tables already holds
+// is shared with it, not built again.  The default tables are built once,
+// before the primary one, and freed with the matrix, so a shared table never
+// outlives the arrays it points to.
+static bool share_table(swift_matrix *M, int W, swift_matrix::SegTable &t) {
+    for (const auto &o : M->cls) {
+        if (&o == &t || !o.seg || o.shared || o.W != W) continue;
+        free_seg_table(t);
+        t = o;
+        t.shared = true;
+        return true;
+    }
+    return false;
 }
 
 static void free_matrix_arrays(swift_matrix *M) {
@@ -1075,17 +1091,19 @@ static int build_seg_table(swift_matrix *M, int W, hipStream_t s, swift_matrix::
 // row table applies).
 int build_segments(swift_matrix *M, int W, hipStream_t s) {
     M->mp_items = W;
+    if (share_table(M, W, M->prim)) return SWIFT_OK;
     return build_seg_table(M, W, s, M->prim);
 }
 
 // The default tables.  A wave's segment of W items should keep enough B
 // bytes in flight and leave enough segments to fill the device: W is the
 // size measured best on the 1M-row benchmark matrix for the class's B rows
-// (48 items for >= 1 KiB rows, 64 for >= 512 B, 128 for 256..511 B, 256 for
-// narrower rows; FP32 K=128 64 items 0.719 ms against 96 items 0.744 ms,
-// BF16 K=128 128 items 0.416 ms against 192 items 0.447 ms), raised to 3
-// mean rows (items per row), which a matrix of long rows (the 200k-node
-// graph: 51 items per row) prefers.  A cut row costs one carry slot.
+// (48 items for >= 1 KiB rows, 64 for >= 512 B, 128 for 256..511 B of 16-bit
+// elements and 96 for wider ones, 128 for narrower rows; FP32 K=128 64 items
+// 0.719 ms against 96 items 0.744 ms, BF16 K=128 128 items 0.416 ms against
+// 192 items 0.447 ms), raised to 3 mean rows (items per row), which a matrix
+// of long rows (the 200k-node graph: 51 items per row) prefers.  A cut row
+// costs one carry slot.  Classes that come to the same W share one table.
 int build_class_segments(swift_matrix *M, hipStream_t s) {
     // measured on the 1M-row matrix with the carry slots and the batch
     // prefetch: FP32 K=32 128 items 0.258 ms against 256 items 0.270 ms, FP64
@@ -1096,8 +1114,8 @@ int build_class_segments(swift_matrix *M, hipStream_t s) {
     const long long m = std::max(M->m, 1);
     const int mean = (int)((m + M->rv_nnz + m - 1) / m);
     for (int c = 0; c < SEG_CLASSES; c++) {
-        SWIFT_TRY(build_seg_table(M, std::min(1024, std::max(by_bytes[c], 3 * mean)), s,
-                                  M->cls[c]));
+        const int W = std::min(1024, std::max(by_bytes[c], 3 * mean));
+        if (!share_table(M, W, M->cls[c])) SWIFT_TRY(build_seg_table(M, W, s, M->cls[c]));
     }
     return SWIFT_OK;
 }
@@ -1201,8 +1219,9 @@ int build_rowview(swift_matrix *M, hipStream_t s) {
         SWIFT_CHECK_LAUNCH();
     }
     SWIFT_HIP(hipStreamSynchronize(s));
-    SWIFT_TRY(build_segments(M, 128, s));
-    return build_class_segments(M, s);
+    // the default tables first: the primary table shares one of the same size
+    SWIFT_TRY(build_class_segments(M, s));
+    return build_segments(M, 128, s);
 }
 
 int row_sort_impl(swift_matrix *M, hipStream_t s) {

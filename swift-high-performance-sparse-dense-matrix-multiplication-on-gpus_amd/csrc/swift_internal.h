@@ -225,6 +225,7 @@ struct swift_matrix {
     struct SegTable {
         int W = 0, nseg = 0, nsplit = 0, nfix = 0;
         int nlong = 0;               // cut rows of more than FIX_LONG slots (first in fix)
+        bool shared = false;         // the arrays belong to a default table of the same W
         int2 *seg = nullptr;         // nseg+1 (row, entry) segment starts
         int *split_rows = nullptr;
         int *seg_slot = nullptr;
