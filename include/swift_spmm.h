@@ -105,8 +105,9 @@ typedef struct {
      * the default tables); for products of tuned_K columns (0: none tuned)
      * the autotuner's walk (SWIFT_KERNEL_DEFAULT = row-view walk,
      * SWIFT_KERNEL_WINDOW = window walk), slot rows per step (0: the
-     * rule) and the widest K slice of one launch (0: no cap; a cap runs K as
-     * several slices of narrower lane groups). */
+     * rule) and the widest K slice of one launch (0: the default, which cuts
+     * B rows of 1 KiB or more of 4/8-byte elements into 512-byte slices; a
+     * cap runs K as several slices of narrower lane groups). */
     int segment_items, tuned_K, tuned_slots;
     swift_kernel tuned_kernel;
     int tuned_cols;
@@ -405,11 +406,12 @@ void swift_batch_destroy(swift_batch *batch);
 
 /* Time the SWIFT_KERNEL_DEFAULT product of one matrix on the caller's
  * operands (C is overwritten) for every candidate schedule and keep the
- * fastest for products of this K and dtype: segment sizes {64, 128, 256,
- * 512} x the slot rows per step the product's lane shape can take (4, 8,
- * 16), the lane shape of half the width (K in two slices), and the window
- * walk (row-major).  best_items receives the segment
- * size; swift_info reports the rest.  Has no counterpart in the reference,
+ * fastest for products of this K and dtype: segment sizes {default, 64,
+ * 128, 256, 512} (default: the table of the B-row bytes) x the slot rows
+ * per step the product's lane shape can take (4, 8, 16), the lane shape of
+ * half the width (K in two slices) and, where the default slices K, the
+ * full width, and the window walk (row-major).  best_items receives the
+ * segment size (0: the default tables); swift_info reports the rest.  Has no counterpart in the reference,
  * whose tiling constants are fixed at compile time
  * (test/k-32/FP32/main.cu:21-23). */
 int swift_autotune(swift_matrix *mat, int K, swift_dtype dtype,

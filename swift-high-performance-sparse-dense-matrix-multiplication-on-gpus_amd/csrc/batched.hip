@@ -176,9 +176,10 @@ static TuneKey tune_key(const swift_matrix *M, int K) {
 }
 
 // Makes c the matrix's schedule for products of K columns with elem-byte B
-// elements: its segment table, slot rows per step and walk.
+// elements: its segment table (W = 0: the default tables by B-row bytes),
+// slot rows per step and walk.
 static int apply_choice(swift_matrix *M, int K, size_t elem, const TuneChoice &c, hipStream_t s) {
-    if (M->mp_items != c.W || !M->user_items) SWIFT_TRY(build_segments(M, c.W, s));
+    if (c.W && (M->mp_items != c.W || !M->user_items)) SWIFT_TRY(build_segments(M, c.W, s));
     M->user_items = c.W;
     M->tuned_K = K;
     M->tuned_elem = (int)elem;
@@ -190,13 +191,15 @@ static int apply_choice(swift_matrix *M, int K, size_t elem, const TuneChoice &c
 
 // Times the SWIFT_KERNEL_DEFAULT product of M on the given operands for every
 // candidate schedule and keeps the fastest (median of 3 after one warm-up).
-// Candidates: segment sizes {64, 128, 256, 512} x the slot-row counts of the
+// Candidates: segment sizes {default, 64, 128, 256, 512} (default: the table
+// of the B-row bytes, W = 0 in the choice) x the slot-row counts of the
 // product's lane shape (slot_candidates), plus at each size the window walk
-// (LDS-staged column windows, row-major only) and the lane shape of half the
-// width (width_candidates: K in two slices of narrower lane groups).
+// (LDS-staged column windows, row-major only) and the other slice widths
+// (width_candidates: half the width, in two slices of narrower lane groups,
+// and full width where the default slices K).
 int autotune_impl(swift_matrix *M, int K, swift_dtype dtype, swift_layout layout, const void *B,
                   void *C, hipStream_t s, TuneChoice *best_out) {
-    static const int sizes[] = {64, 128, 256, 512};
+    static const int sizes[] = {0, 64, 128, 256, 512};
     const size_t elem = value_size(dtype);
     std::vector<TuneChoice> cands;
     const int vec = vec_of(elem);
@@ -432,7 +435,7 @@ static int tune_batch_key(swift_matrix *const *mats, const int *Ks, const std::v
                           hipStream_t s, TuneChoice *best_out) {
     const int K = Ks[idx[0]];
     std::vector<int> slots{0, batch_alt_slots(K)};
-    std::vector<int> widths = width_candidates(K, sizeof(float), true);
+    std::vector<int> widths = width_candidates(K, sizeof(float), true, false);
     if (g_batch_width == SWIFT_BATCH_WIDTH_FULL) {
         widths = {0};
     } else if (g_batch_width == SWIFT_BATCH_WIDTH_HALF && widths.size() > 1) {

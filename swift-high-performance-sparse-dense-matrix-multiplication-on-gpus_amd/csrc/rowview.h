@@ -702,7 +702,9 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 // One-chunk 64-lane groups of 4- and 8-byte elements (FP32 K=256, FP64
 // K=128) take 16 slots: on the 1M-row matrix FP32 K=256 1.550 -> 1.519 ms,
 // FP64 K=128 1.531 -> 1.523 ms; BF16 K=256 was slower (0.896 -> 0.904 ms)
-// and keeps 8.
+// and keeps 8.  16-lane groups of 2-byte elements (BF16/F16 K=64) take 8:
+// BF16 K=64 0.297 -> 0.288 ms on the 1M-row matrix, 0.139 -> 0.132 ms on the
+// 200k-node graph; the 8-lane groups (K=32) keep 4 (0.265 -> 0.270 ms at 8).
 
 //
 // WIN: the workgroup has staged a window of B rows in LDS (win).  A step
@@ -717,10 +719,11 @@ __device__ __forceinline__ void gather_step(const RowArgs &a, __amdgpu_buffer_rs
 // SLOTS: slot rows (B-row loads in flight per lane group) per step; 0 takes
 // rule_slots, other values come from the autotuner.
 constexpr int rule_slots(int lpe, int q, int elem, bool win = false) {
-    return (lpe == 64 && q == 1 && elem >= 4 && !win) ? 16
-           : (lpe >= 32 && q <= 2 && !win) ? SWIFT_UNROLL_LONG
-           : elem == 2                   ? SWIFT_UNROLL_NARROW
-                                         : SWIFT_UNROLL_WIDE;
+    return (lpe == 64 && q == 1 && elem >= 4 && !win)      ? 16
+           : (lpe >= 32 && q <= 2 && !win)                ? SWIFT_UNROLL_LONG
+           : (lpe == 16 && q == 1 && elem == 2 && !win) ? 8
+           : elem == 2                                  ? SWIFT_UNROLL_NARROW
+                                                        : SWIFT_UNROLL_WIDE;
 }
 
 // Slot-row counts the autotuner may choose for a lane shape: one chunk per
@@ -732,17 +735,37 @@ constexpr bool slots_ok(int lpe, int q, int slots) {
 // Elements per lane load for B elements of elem bytes (Vec16 on the host).
 inline int vec_of(size_t elem) { return elem == 8 ? 2 : elem == 4 ? 4 : SWIFT_NARROW_VEC; }
 
-// Column caps worth timing for a product of K columns (0: none): half the
-// width of the first full-vector slice, when that still is a full-vector
-// shape of at least 8 lanes.
-inline std::vector<int> width_candidates(int K, size_t elem, bool aligned) {
+// Widest K slice of one launch of the default single product (0: no cap):
+// B rows of 1 KiB or more of 4- and 8-byte elements run in slices of 512
+// bytes, each a pass over A with the 32-lane (FP32) or 64-lane (FP64) shape
+// of one 512-byte chunk.  A narrower slice holds more B rows of a segment
+// window in L2 for the same bytes.  Measured on the 1M-row matrix (ms, full
+// width -> 512-byte slices): FP32 K=256 1.513 -> 1.439, K=512 3.266 ->
+// 2.874 (1 KiB slices 3.024); FP64 K=128 1.518 -> 1.477, K=256 3.279 ->
+// 2.951; column-major FP32 K=256 2.290 -> 2.213; the 200k-node graph FP32
+// K=256 1.180 -> 1.101.  BF16 rows are not cut: 256-byte slices of BF16
+// K=256 / 512 took 0.894 -> 0.904 / 1.849 -> 2.003 ms on the 1M-row matrix
+// (and 0.543 -> 0.491 ms on the 200k-node graph, where the autotuner finds
+// them).  The segment table follows the bytes of the first slice.
+inline int default_slice_cols(int K, size_t elem) {
+    return (elem >= 4 && (size_t)K * elem >= 1024) ? (int)(512 / elem) : 0;
+}
+
+// Column caps worth timing for a product of K columns (0: the default, for
+// a single product default_slice_cols, for a batch no cap): half the width
+// of the first uncapped full-vector slice, when that still is a full-vector
+// shape of at least 8 lanes, and for a single product whose default caps the
+// width, the uncapped width itself.
+inline std::vector<int> width_candidates(int K, size_t elem, bool aligned, bool single = true) {
     std::vector<int> out{0};
     const int vec = vec_of(elem);
+    const int rule = single ? default_slice_cols(K, elem) : 0;
     for (auto &sl : plan_row_slices(K, vec, aligned)) {
         const RowTarget &t = sl.second;
         if (t.mask) break;
         const int cols = vec * t.lpe * t.q;
-        if (cols / 2 >= vec * 8) out.push_back(cols / 2);
+        if (rule && rule < cols) out.push_back(cols);
+        if (cols / 2 >= vec * 8 && cols / 2 != rule) out.push_back(cols / 2);
         break;
     }
     return out;

@@ -613,8 +613,21 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.rrow = M->reside_cscrowidx;
     a.rval = rval;
     a.m = M->m;
-    // the default table of the B row size, unless the size was set
-    const swift_matrix::SegTable &tb = M->cls[seg_class((size_t)K * sizeof(TB), sizeof(TB))];
+    constexpr int V = Vec16<TB>::N;
+    const bool aligned = (K % V) == 0 && ((uintptr_t)B % 16) == 0 && ((uintptr_t)C % 16) == 0;
+    // the autotuner's choice for this K and element size (DEFAULT only)
+    const bool tuned = kernel == SWIFT_KERNEL_DEFAULT && M->tuned_K == K &&
+                       M->tuned_elem == (int)sizeof(TB);
+    // the default product's slice width: the autotuner's cap, else the rule
+    const int cap = tuned && M->tuned_cols ? M->tuned_cols
+                    : kernel == SWIFT_KERNEL_DEFAULT ? default_slice_cols(K, sizeof(TB))
+                                                     : 0;
+    const auto slices = plan_row_slices(K, V, aligned, cap);
+    // the default table of the bytes one entry gathers in the first slice,
+    // unless the size was set
+    const RowTarget &t0 = slices.front().second;
+    const int cols0 = t0.mask ? std::min(64, K) : t0.vec * t0.lpe * t0.q;
+    const swift_matrix::SegTable &tb = M->cls[seg_class((size_t)cols0 * sizeof(TB), sizeof(TB))];
     const swift_matrix::SegTable &st = !M->user_items && tb.seg ? tb : M->prim;
     a.nseg = st.nseg;
     a.seg = st.seg;
@@ -633,12 +646,6 @@ static int spmm_row(const swift_matrix *M, int K, swift_kernel kernel, unsigned 
     a.ldc = K;
     a.kscale = kscale;
     a.gscale = gscale;
-    constexpr int V = Vec16<TB>::N;
-    const bool aligned = (K % V) == 0 && ((uintptr_t)B % 16) == 0 && ((uintptr_t)C % 16) == 0;
-    // the autotuner's choice for this K and element size (DEFAULT only)
-    const bool tuned = kernel == SWIFT_KERNEL_DEFAULT && M->tuned_K == K &&
-                       M->tuned_elem == (int)sizeof(TB);
-    const auto slices = plan_row_slices(K, V, aligned, tuned ? M->tuned_cols : 0);
     a.slots = tuned ? M->tuned_slots : 0;
     if (tuned && M->tuned_kernel == SWIFT_KERNEL_WINDOW) kernel = SWIFT_KERNEL_WINDOW;
     const bool flat = kernel == SWIFT_KERNEL_FLAT;

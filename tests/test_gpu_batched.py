@@ -161,7 +161,7 @@ def test_autotune_single(swift, oracle, cuda):
     B = torch.tensor(b, device="cuda")
     C = torch.empty((m, 64), device="cuda")
     W = M.autotune(B, C)
-    assert W in (64, 128, 256, 512)
+    assert W in (0, 64, 128, 256, 512)  # 0: the default table of the B-row bytes
     C = M.spmm(B)
     torch.cuda.synchronize()
     C64, absC = oracle.spmm_f64(m, n, rp, ci, v32.astype(np.float64), b.astype(np.float64))
@@ -170,15 +170,19 @@ def test_autotune_single(swift, oracle, cuda):
 
 
 # lane shape of the first K slice -> slot-row counts the autotuner may pick
-# (0 = the rule: 8 for 32/64-lane groups, 4 below; 16 only where a step still
-# fits one staged batch of 64 entries)
-SLOT_CHOICES = {8: {0, 8}, 16: {0, 8, 16}, 32: {0, 4, 16}, 64: {0, 4, 16}}
+# (0 = the rule: 16 for one-chunk 64-lane groups of 4/8-byte elements, 8 for
+# other 32/64-lane groups and 16-lane groups of 2-byte elements, 4 below; 16
+# only where a step still fits one staged batch of 64 entries)
+SLOT_CHOICES = {8: {0, 8}, 16: {0, 4, 8, 16}, 32: {0, 4, 16}, 64: {0, 4, 8, 16}}
 
 
-@pytest.mark.parametrize("dt,K,lpe,half", [("f32", 256, 64, 128), ("bf16", 64, 16, 32),
-                                           ("f32", 100, 16, 32), ("f64", 32, 16, 16),
-                                           ("f32", 128, 32, 64)])
-def test_autotune_schedule_applied(swift, oracle, cuda, dt, K, lpe, half):
+# (dtype, K, lane group of the first uncapped K slice, slice caps the autotuner
+# may pick: 0 = the default, half the first slice's width, and where the
+# default cuts B rows of 1 KiB or more into 512-byte slices, the full width)
+@pytest.mark.parametrize("dt,K,lpe,widths", [("f32", 256, 64, (0, 256)), ("bf16", 64, 16, (0, 32)),
+                                             ("f32", 100, 16, (0, 32)), ("f64", 32, 16, (0, 16)),
+                                             ("f32", 128, 32, (0, 64))])
+def test_autotune_schedule_applied(swift, oracle, cuda, dt, K, lpe, widths):
     """swift_autotune over segment size x slot rows x lane width x walk on the benchmark
     structure at 1/10 scale: the choice is reported by swift_info, taken by
     the DEFAULT product of that K (before and after another K's product) and
@@ -201,11 +205,12 @@ def test_autotune_schedule_applied(swift, oracle, cuda, dt, K, lpe, half):
     W = M.autotune(B, C)
     inf = M.info()
     print("tuned", W, inf.tuned_K, inf.tuned_slots, inf.tuned_kernel, inf.tuned_cols, flush=True)
-    assert inf.segment_items == W and W in (64, 128, 256, 512) and inf.tuned_K == K
+    # W = 0: the default table of the B-row bytes
+    assert inf.segment_items == W and W in (0, 64, 128, 256, 512) and inf.tuned_K == K
     assert inf.tuned_kernel in (swift.KERNEL_DEFAULT, swift.KERNEL_WINDOW)
     assert inf.tuned_slots in SLOT_CHOICES[lpe]
-    # a width cap halves the first slice's lane group (K in more launches)
-    assert inf.tuned_cols in (0, half)
+    # a width cap narrows the first slice's lane group (K in more launches)
+    assert inf.tuned_cols in widths
     if inf.tuned_kernel == swift.KERNEL_WINDOW or inf.tuned_cols:
         assert inf.tuned_slots == 0
     C64, absC = oracle.spmm_f64(m, n, rp, ci, v.astype(np.float64), b.astype(np.float64))
