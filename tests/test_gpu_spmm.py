@@ -114,9 +114,10 @@ def test_b_sorted_flag(swift, oracle, cuda, layout, dt):
 
 
 # Which entries the row view holds decides the launches of a row-view product:
-# powerlaw_3000x2500 (reside part under 1% of its entries, spread over more
-# than 1 row in 64) walks every entry in the row view; tall_2000x64 (12%
-# reside) keeps the reside blocks in a launch of their own after the walk.
+# powerlaw_3000x2500 (reside part under 1% of its entries) walks every entry
+# in the row view; tall_2000x64 (12% reside) runs its reside blocks after the
+# walk (in the tail launch of the row-view walks, in a launch of their own
+# after the flat walk).
 ROWVIEW_HOLDS_ALL = {"powerlaw_3000x2500": True, "tall_2000x64": False}
 
 
