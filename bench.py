@@ -414,7 +414,8 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def time_call(torch, fn, reps=10, warm=2):
+def time_call(torch, fn, reps=10, warm=5):
+    # the untimed calls first let the clocks settle after the builds between legs
     for _ in range(warm):
         fn()
     torch.cuda.synchronize()
