@@ -703,12 +703,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
             continue;
         }
         if (last && !cut_end) break;
-#pragma unroll
-        for (int o = LPE; o < 64; o <<= 1)
-#pragma unroll
-            for (int q = 0; q < Q; q++)
-#pragma unroll
-                for (int v = 0; v < VEC; v++) acc[q][v] += __shfl_xor(acc[q][v], o, 64);
+        subgroup_sum<T, VEC, Q, LPE>(acc);
         if (h == 0) {
             // the piece of a row this segment starts inside goes to its carry
             // slot; every other row (and the first piece of a cut row) to dB
