@@ -598,6 +598,31 @@ __device__ __forceinline__ double xor_add(double x) {
     }
 }
 
+// The value of x held by lane (lane ^ o), every lane active, for o a power
+// of two below 64 that is known once the caller's loops unroll: o = 16 / 32
+// by the lane swaps (the partner's value is the result that is not the
+// lane's own), 8 by DPP row_ror:8, 2 and 1 by DPP quad permutations, 4 by a
+// shuffle.
+__device__ __forceinline__ unsigned lane_xor_u32(unsigned u, int o) {
+    if (o == 32 || o == 16) {
+        const auto r = o == 32 ? __builtin_amdgcn_permlane32_swap(u, u, false, false)
+                               : __builtin_amdgcn_permlane16_swap(u, u, false, false);
+        return (threadIdx.x & (unsigned)o) ? r[0] : r[1];
+    }
+    if (o == 8) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x128, 0xf, 0xf, false);
+    if (o == 2) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x4E, 0xf, 0xf, false);
+    if (o == 1) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0xB1, 0xf, 0xf, false);
+    return (unsigned)__shfl_xor((int)u, o, 64);
+}
+__device__ __forceinline__ float lane_xor(float x, int o) {
+    return __uint_as_float(lane_xor_u32(__float_as_uint(x), o));
+}
+__device__ __forceinline__ double lane_xor(double x, int o) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = lane_xor_u32((unsigned)u, o), hi = lane_xor_u32((unsigned)(u >> 32), o);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // The row-end butterfly: sums the EPS = 64 / LPE sub-groups' partial sums,
 // every lane ending with the total of its sub-lane.  With the lane exchanges
 // above in place of shuffles: on the 1M-row matrix FP32 K=32 0.258 -> 0.239
