@@ -611,7 +611,7 @@ __global__ __launch_bounds__(256) void k_bw_walk(BwArgs a) {
             rbase = row;
             rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
         }
-        const int rend = __builtin_amdgcn_readfirstlane(__shfl(rend_l, row - rbase, 64));
+        const int rend = __builtin_amdgcn_readlane(rend_l, row - rbase);
         const int row_end = row < r1 ? rend : e1;
         if (DV && e < row_end) {
 #pragma unroll

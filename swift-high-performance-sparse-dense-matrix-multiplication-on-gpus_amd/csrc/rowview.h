@@ -940,7 +940,7 @@ __device__ __forceinline__ void rowview_walk(const RowArgs &a, int sid, Window w
             rend_l = (rbase + lane < a.m) ? ld_stream(a.rv_ptr + rbase + lane + 1) : e1;
         }
         // wave-uniform: kept in scalar registers so the slot tests below branch
-        const int rend = __builtin_amdgcn_readfirstlane(__shfl(rend_l, row - rbase, 64));
+        const int rend = __builtin_amdgcn_readlane(rend_l, row - rbase);
         const int row_end = row < r1 ? rend : e1;
         while (e < row_end) {
             if (e - ebase >= 64) {
