@@ -837,21 +837,20 @@ inline int default_slice_cols(int K, size_t elem) {
     return (elem >= 4 && (size_t)K * elem >= 1024) ? (int)(512 / elem) : 0;
 }
 
-// Column caps worth timing for a product of K columns (0: the default, for
-// a single product default_slice_cols, for a batch no cap): half the width
-// of the first uncapped full-vector slice, when that still is a full-vector
-// shape of at least 8 lanes, and for a single product whose default caps the
-// width, the uncapped width itself.
+// Column caps worth timing for a product of K columns (0: the default -- for
+// a single product the default slicing rules, which may cut K; for a batch
+// no cap): half the width of the first uncapped full-vector slice, when that
+// still is a full-vector shape of at least 8 lanes, and for a single product
+// the uncapped width itself.
 inline std::vector<int> width_candidates(int K, size_t elem, bool aligned, bool single = true) {
     std::vector<int> out{0};
     const int vec = vec_of(elem);
-    const int rule = single ? default_slice_cols(K, elem) : 0;
     for (auto &sl : plan_row_slices(K, vec, aligned)) {
         const RowTarget &t = sl.second;
         if (t.mask) break;
         const int cols = vec * t.lpe * t.q;
-        if (rule && rule < cols) out.push_back(cols);
-        if (cols / 2 >= vec * 8 && cols / 2 != rule) out.push_back(cols / 2);
+        if (single) out.push_back(cols);
+        if (cols / 2 >= vec * 8) out.push_back(cols / 2);
         break;
     }
     return out;

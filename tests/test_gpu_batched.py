@@ -177,11 +177,12 @@ SLOT_CHOICES = {8: {0, 8}, 16: {0, 4, 8, 16}, 32: {0, 4, 16}, 64: {0, 4, 8, 16}}
 
 
 # (dtype, K, lane group of the first uncapped K slice, slice caps the autotuner
-# may pick: 0 = the default, half the first slice's width, and where the
-# default cuts B rows of 1 KiB or more into 512-byte slices, the full width)
-@pytest.mark.parametrize("dt,K,lpe,widths", [("f32", 256, 64, (0, 256)), ("bf16", 64, 16, (0, 32)),
-                                             ("f32", 100, 16, (0, 32)), ("f64", 32, 16, (0, 16)),
-                                             ("f32", 128, 32, (0, 64))])
+# may pick: 0 = the default, the first uncapped slice's full width and half)
+@pytest.mark.parametrize("dt,K,lpe,widths", [("f32", 256, 64, (0, 256, 128)),
+                                             ("bf16", 64, 16, (0, 64, 32)),
+                                             ("f32", 100, 16, (0, 64, 32)),
+                                             ("f64", 32, 16, (0, 32, 16)),
+                                             ("f32", 128, 32, (0, 128, 64))])
 def test_autotune_schedule_applied(swift, oracle, cuda, dt, K, lpe, widths):
     """swift_autotune over segment size x slot rows x lane width x walk on the benchmark
     structure at 1/10 scale: the choice is reported by swift_info, taken by
